@@ -1,0 +1,208 @@
+"""Benchmark: SpGEMM GFLOP/s and HBM-roofline fraction on one MI355X.
+
+Workload (one "step"): C = A*A with A the R-MAT matrix of scale 20, edge
+factor 16, quadrant probabilities .57/.19/.19/.05, duplicates merged,
+self-loops kept, FP32 values U(-1,1), seed 1 -- inputs resident in HBM when
+the timed region starts, C complete in HBM (CSR) when it ends.
+
+Work unit: intermediate products P = sum over A(i,k) of nnz(B(k,:));
+flops = 2P; value = 2P / t in GFLOP/s.  Algorithmic bytes of the whole
+product (A read once, C written once, CSR with int64 row pointers and
+4+4-byte entries): 8(m+1) + 8 nnz(A) + 8(m+1) + 8 nnz(C).
+
+Prints ONE JSON line on stdout (human notes go to stderr).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SpGEMM GFLOP/s & % HBM roofline: learned+batched vs plain two-phase, 1xMI355X"
+HBM_PEAK_GBS = 8000.0
+TH = [16, 64, 256, 1024, 4096]  # default row-bin thresholds of the library
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def row_products(A):
+    lens = np.diff(A.row_ptr)
+    return np.add.reduceat(np.append(lens[A.col], 0), A.row_ptr[:-1]) * (np.diff(A.row_ptr) > 0)
+
+
+def kernel_bytes(name, A, P, cnnz, ncols):
+    """Algorithmic bytes of one launch of a named kernel: row pointers (16 B
+    per row), A entries it must read (col 4 B, +val 4 B for numeric), C
+    entries it writes (8 B) or row counts (8 B)."""
+    anz = np.diff(A.row_ptr)
+    U = np.minimum(P, ncols)
+
+    def binmask(key, b):
+        lo = 0 if b == 0 else TH[b - 1]
+        hi = TH[b] if b < 5 else np.iinfo(np.int64).max
+        return (key > lo) & (key <= hi)
+
+    sizes = {"t32": 0, "t128": 1, "t512": 2, "t2048": 3, "t8192": 4}
+    if name.startswith("num_hash_"):
+        m = binmask(cnnz, sizes[name.split("_")[-1]])
+        return float(16 * m.sum() + 8 * anz[m].sum() + 8 * cnnz[m].sum())
+    if name == "num_tile":
+        m = cnnz > TH[4]
+        return float(16 * m.sum() + 8 * anz[m].sum() + 8 * cnnz[m].sum())
+    if name.startswith("sym_hash_"):
+        m = binmask(U, sizes[name.split("_")[-1]])
+        return float(24 * m.sum() + 4 * anz[m].sum())
+    if name == "sym_bitmap":
+        m = U > TH[4]
+        return float(24 * m.sum() + 4 * anz[m].sum())
+    return None
+
+
+def cpu_baseline(A, P, threads, target_s=15.0):
+    """FP32 Gustavson (reference semantics) of a strided row sample of the
+    same product, on `threads` host cores; returns GFLOP/s and the sample."""
+    import oracle
+    m = A.shape[0]
+
+    def sample(stride):
+        idx = np.arange(0, m, stride)
+        lens = np.diff(A.row_ptr)[idx]
+        rp = np.zeros(idx.size + 1, np.int64)
+        np.cumsum(lens, out=rp[1:])
+        starts = A.row_ptr[idx]
+        flat = np.repeat(starts - rp[:-1], lens) + np.arange(rp[-1])
+        from rtspmspm_amd import CSR
+        return CSR((idx.size, A.shape[1]), rp, A.col[flat], A.val[flat]), float(P[idx].sum())
+
+    stride = 4096
+    while True:
+        S, ps = sample(stride)
+        t0 = time.perf_counter()
+        oracle.stream_f32(S, A, 0, S.shape[0], threads)
+        t = time.perf_counter() - t0
+        if t > target_s / 8 or stride == 1:
+            break
+        stride = max(1, int(stride * max(t, 1e-3) / target_s * 2) if t > 0.05 else stride // 16)
+        stride = max(1, stride)
+    if t < target_s / 2 and stride > 1:
+        stride = max(1, int(stride * t / target_s))
+        S, ps = sample(stride)
+        t0 = time.perf_counter()
+        oracle.stream_f32(S, A, 0, S.shape[0], threads)
+        t = time.perf_counter() - t0
+    return {"value": 2.0 * ps / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th row of A ({S.shape[0]} rows, {ps:.3e} products), "
+                      f"FP32 Gustavson with sorted CSR rows, computed and discarded, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--method", default="plain", choices=["plain", "learned", "ray"])
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU row "
+              "partition is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    if "RANK" in os.environ and int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        sys.exit("bench.py is single-GPU")
+
+    import rtspmspm_amd as rt
+    dev = rt.device_info()
+    log(f"device {dev['name']}, {dev['cus']} CUs, {dev['total_mem'] / 2**30:.0f} GiB")
+    t0 = time.perf_counter()
+    A = rt.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1)
+    P = row_products(A)
+    ptot = int(P.sum())
+    log(f"R-MAT s{args.scale}: n={A.shape[0]} nnz={A.nnz} products={ptot} "
+        f"(gen {time.perf_counter() - t0:.1f} s)")
+    dA = rt.to_device(A)
+
+    ms_dev = []
+    C = None
+    for _ in range(args.warmup):
+        C, info = rt.spmspm(dA, dA, mode=args.method)
+        C.free()
+    rt.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        C, info = rt.spmspm(dA, dA, mode=args.method)
+        ms_dev.append(info["ms"])
+        if s + 1 < args.steps:
+            C.free()
+    rt.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    cnnz_row = np.diff(C.row_ptr_host())
+    nnz_c = int(cnnz_row.sum())
+    C.free()
+    m = A.shape[0]
+    bytes_alg = 8 * (m + 1) + 8 * A.nnz + 8 * (m + 1) + 8 * nnz_c
+    gflops = 2.0 * ptot / (ms_per_step / 1e3) / 1e9
+    log(f"{args.method}: {ms_per_step:.2f} ms/step (device {np.median(ms_dev):.2f} ms median), "
+        f"nnz(C)={nnz_c}, {gflops:.1f} GFLOP/s, pipeline {bytes_alg / ms_per_step / 1e6:.1f} GB/s")
+
+    roofline = None
+    kernels = {}
+    if not args.no_profile:
+        rt.prof_reset()
+        C, _ = rt.spmspm(dA, dA, mode=args.method, profile=True)
+        C.free()
+        stats = rt.prof_stats()
+        kernels = {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in stats.items()}
+        dom = max(stats, key=lambda k: stats[k][1])
+        launches, tot_ms = stats[dom]
+        avg_s = tot_ms / launches / 1e3
+        kb = kernel_bytes(dom, A, P, cnnz_row, A.shape[1])
+        achieved = kb / avg_s / 1e9 if kb else None
+        traffic = None
+        pmc = os.path.join(ROOT, "profiles", f"{dom}_rmat{args.scale}_traffic.json")
+        if os.path.exists(pmc):
+            with open(pmc) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                    "traffic": traffic, "alg_bytes_per_launch": kb,
+                    "avg_launch_ms": avg_s * 1e3}
+        log("kernels: " + ", ".join(f"{k} {v['ms']:.2f} ms/{v['launches']}" for k, v in
+                                    sorted(kernels.items(), key=lambda kv: -kv[1]["ms"])))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(A, P, threads)
+        log(f"cpu baseline: {cpu['value']:.2f} GFLOP/s on {threads} threads ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC, "value": round(gflops, 3), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale}, edge factor 16, a/b/c/d .57/.19/.19/.05, "
+                               "seed 1, C = A*A", "method": args.method, "rows": m,
+                   "nnz_A": int(A.nnz), "products": ptot, "nnz_C": nnz_c,
+                   "bytes_alg": bytes_alg,
+                   "pipeline_gbs": round(bytes_alg / ms_per_step / 1e6, 2),
+                   "pipeline_roofline_frac": round(bytes_alg / ms_per_step / 1e6 / HBM_PEAK_GBS, 5),
+                   "device_ms_median": round(float(np.median(ms_dev)), 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

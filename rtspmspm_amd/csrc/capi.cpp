@@ -1,0 +1,514 @@
+// extern "C" boundary of librtspmspm (declared in include/rt_spmspm.h).
+// Every entry point catches the library's exceptions and returns an RT_E*
+// code; rt_last_error() carries the message (the reference's equivalent is
+// the CUDA_CHECK/OPTIX_CHECK exception caught in main, optixSpMSpM.cpp:654-658).
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "hostcsr.hpp"
+#include "learned.hpp"
+#include "spgemm.hpp"
+
+namespace rt {
+const char* last_error_cstr();
+void prof_reset();
+int prof_count();
+bool prof_get(int i, std::string& name, i64& launches, double& ms);
+void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
+                      const char* what);
+}  // namespace rt
+
+using namespace rt;
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+  try {
+    set_last_error("");
+    f();
+    return RT_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return RT_E_NOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return RT_E_INVALID;
+  }
+}
+
+void zero_csr(rt_csr* C) {
+  if (C) std::memset(C, 0, sizeof(*C));
+}
+
+void check_csr_header(const rt_csr* M, const char* name) {
+  RT_CHECK(M != nullptr, RT_E_INVALID, std::string(name) + " is NULL");
+  RT_CHECK(M->rows >= 0 && M->cols >= 0 && M->nnz >= 0, RT_E_INVALID,
+           std::string(name) + " has negative dimensions");
+  RT_CHECK(M->mem == RT_MEM_HOST || M->mem == RT_MEM_DEVICE, RT_E_INVALID,
+           std::string(name) + " has an unknown memory space");
+  RT_CHECK(M->row_ptr != nullptr, RT_E_INVALID, std::string(name) + ".row_ptr is NULL");
+  RT_CHECK(M->nnz == 0 || (M->col != nullptr && M->val != nullptr), RT_E_INVALID,
+           std::string(name) + " has NULL col/val");
+  RT_CHECK(M->cols <= (i64)INT32_MAX, RT_E_LIMIT, std::string(name) + " has more than 2^31-1 columns");
+}
+
+// Full structural check of a host CSR: row_ptr monotone from 0 to nnz,
+// columns in range and strictly increasing inside each row.
+void validate_host(const rt_csr* M, const char* name) {
+  RT_CHECK(M->row_ptr[0] == 0 && M->row_ptr[M->rows] == M->nnz, RT_E_INVALID,
+           std::string(name) + ": row_ptr must run from 0 to nnz");
+  for (i64 i = 0; i < M->rows; ++i) {
+    const i64 a = M->row_ptr[i], b = M->row_ptr[i + 1];
+    RT_CHECK(a <= b, RT_E_INVALID, std::string(name) + ": row_ptr not monotone");
+    for (i64 e = a; e < b; ++e) {
+      RT_CHECK(M->col[e] >= 0 && M->col[e] < M->cols, RT_E_INVALID,
+               std::string(name) + ": column index out of range");
+      RT_CHECK(e == a || M->col[e] > M->col[e - 1], RT_E_INVALID,
+               std::string(name) + ": columns must be sorted and unique within a row");
+    }
+  }
+}
+
+Csr view(const rt_csr* M) {
+  Csr v;
+  v.rows = M->rows;
+  v.cols = M->cols;
+  v.nnz = M->nnz;
+  v.rp = M->row_ptr;
+  v.col = M->col;
+  v.val = M->val;
+  return v;
+}
+
+struct DevCopy {
+  DBuf<i64> rp;
+  DBuf<i32> col;
+  DBuf<float> val;
+  Csr v;
+  DevCopy(const rt_csr* M, hipStream_t s) {
+    rp.alloc(M->rows + 1);
+    col.alloc(std::max<i64>(M->nnz, 1));
+    val.alloc(std::max<i64>(M->nnz, 1));
+    RT_HIP(hipMemcpyAsync(rp.p, M->row_ptr, (M->rows + 1) * sizeof(i64), hipMemcpyHostToDevice, s));
+    if (M->nnz) {
+      RT_HIP(hipMemcpyAsync(col.p, M->col, M->nnz * sizeof(i32), hipMemcpyHostToDevice, s));
+      RT_HIP(hipMemcpyAsync(val.p, M->val, M->nnz * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    v.rows = M->rows;
+    v.cols = M->cols;
+    v.nnz = M->nnz;
+    v.rp = rp.p;
+    v.col = col.p;
+    v.val = val.p;
+  }
+};
+
+void free_out(CsrOut& C) {
+  dfree(C.rp);
+  dfree(C.col);
+  dfree(C.val);
+  C = CsrOut{};
+}
+
+void export_device(CsrOut& Cd, rt_csr* C) {
+  C->rows = Cd.rows;
+  C->cols = Cd.cols;
+  C->nnz = Cd.nnz;
+  C->row_ptr = Cd.rp;
+  C->col = Cd.col;
+  C->val = Cd.val;
+  C->mem = RT_MEM_DEVICE;
+  C->owner = 1;
+  Cd = CsrOut{};
+}
+
+void export_host(CsrOut& Cd, rt_csr* C, hipStream_t s) {
+  HostCsr h;
+  h.alloc(Cd.rows, Cd.cols, Cd.nnz);
+  RT_HIP(hipMemcpyAsync(h.rp, Cd.rp, (Cd.rows + 1) * sizeof(i64), hipMemcpyDeviceToHost, s));
+  if (Cd.nnz) {
+    RT_HIP(hipMemcpyAsync(h.col, Cd.col, Cd.nnz * sizeof(i32), hipMemcpyDeviceToHost, s));
+    RT_HIP(hipMemcpyAsync(h.val, Cd.val, Cd.nnz * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  RT_HIP(hipStreamSynchronize(s));
+  free_out(Cd);
+  h.export_to(C);
+}
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  double ms() {
+    RT_HIP(hipEventSynchronize(b));
+    float t = 0;
+    RT_HIP(hipEventElapsedTime(&t, a, b));
+    return t;
+  }
+};
+
+// Runs the selected method on device operands; C receives pool arrays.
+void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts& o,
+                hipStream_t s) {
+  RunStats st;
+  EventPair ev;
+  Prof::enabled = o.profile != 0;
+  RT_HIP(hipEventRecord(ev.a, s));
+  try {
+    switch (o.mode) {
+      case RT_MODE_PLAIN: spgemm_plain(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st); break;
+      case RT_MODE_RAY:
+        RT_CHECK(bt.b_base == nullptr, RT_E_INVALID, "the ray method takes a single product");
+        spgemm_ray(A, B, C, o.ablation, o.ray_output, s, &st);
+        break;
+      case RT_MODE_LEARNED:
+        spgemm_learned(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st);
+        break;
+      default: fail(RT_E_INVALID, "unknown mode " + std::to_string(o.mode));
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(s);  // drain before pool blocks are reused
+    free_out(C);
+    Prof::enabled = false;
+    throw;
+  }
+  RT_HIP(hipEventRecord(ev.b, s));
+  o.ms_total = ev.ms();
+  if (Prof::enabled) Prof::flush();
+  Prof::enabled = false;
+  o.products = st.products;
+  o.overflow_rows = st.overflow_rows;
+  o.predicted_rows = st.predicted_rows;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rt_last_error(void) { return last_error_cstr(); }
+
+int rt_spmspm(const rt_csr* A, const rt_csr* B, rt_csr* C) {
+  return rt_spmspm_ex(A, B, C, nullptr, nullptr);
+}
+
+int rt_spmspm_ex(const rt_csr* A, const rt_csr* B, rt_csr* C, rt_opts* opts, void* stream) {
+  zero_csr(C);
+  return guard([&] {
+    RT_CHECK(C != nullptr, RT_E_INVALID, "C is NULL");
+    check_csr_header(A, "A");
+    check_csr_header(B, "B");
+    RT_CHECK(A->cols == B->rows, RT_E_SHAPE,
+             "shape mismatch: A is " + std::to_string(A->rows) + "x" + std::to_string(A->cols) +
+                 ", B is " + std::to_string(B->rows) + "x" + std::to_string(B->cols));
+    RT_CHECK(A->mem == B->mem, RT_E_INVALID, "A and B must live in the same memory space");
+    rt_opts local{};
+    rt_opts& o = opts ? *opts : local;
+    device();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    CsrOut Cd;
+    if (A->mem == RT_MEM_HOST) {
+      validate_host(A, "A");
+      validate_host(B, "B");
+      DevCopy dA(A, s), dB(B, s);
+      run_device(dA.v, dB.v, Batch{}, Cd, o, s);
+      export_host(Cd, C, s);
+    } else {
+      validate_csr_dev(view(A), A->cols, nullptr, s, "A");
+      validate_csr_dev(view(B), B->cols, nullptr, s, "B");
+      run_device(view(A), view(B), Batch{}, Cd, o, s);
+      export_device(Cd, C);
+    }
+  });
+}
+
+int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_rows,
+                      const int64_t* b_cols, const rt_csr* A, const rt_csr* B, rt_csr* C,
+                      rt_opts* opts, void* stream) {
+  zero_csr(C);
+  return guard([&] {
+    RT_CHECK(C != nullptr && a_rows && b_rows && b_cols, RT_E_INVALID, "NULL argument");
+    RT_CHECK(count >= 1, RT_E_INVALID, "count must be >= 1");
+    check_csr_header(A, "A_stack");
+    check_csr_header(B, "B_stack");
+    RT_CHECK(A->mem == RT_MEM_DEVICE && B->mem == RT_MEM_DEVICE, RT_E_INVALID,
+             "batched operands must be device memory");
+    i64 ar = 0, br = 0, maxk = 0, maxc = 0;
+    for (int b = 0; b < count; ++b) {
+      RT_CHECK(a_rows[b] >= 0 && b_rows[b] >= 0 && b_cols[b] >= 0, RT_E_INVALID, "negative pair size");
+      ar += a_rows[b];
+      br += b_rows[b];
+      maxk = std::max<i64>(maxk, b_rows[b]);
+      maxc = std::max<i64>(maxc, b_cols[b]);
+    }
+    RT_CHECK(ar == A->rows && br == B->rows, RT_E_SHAPE, "stacked row counts do not match the pair sizes");
+    RT_CHECK(A->cols >= maxk && B->cols >= maxc, RT_E_SHAPE, "stacked column counts too small");
+    rt_opts local{};
+    rt_opts& o = opts ? *opts : local;
+    RT_CHECK(o.mode == RT_MODE_PLAIN || o.mode == RT_MODE_LEARNED, RT_E_INVALID,
+             "batched products support the plain and learned methods");
+    device();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    // Per-row B base and per-row column limits of both operands.
+    DBuf<i64> base(std::max<i64>(ar, 1)), alim(std::max<i64>(ar, 1)), blim(std::max<i64>(br, 1));
+    {
+      std::vector<i64> bstart(count), av(count), bv(count);
+      i64 acc = 0;
+      for (int b = 0; b < count; ++b) {
+        bstart[b] = acc;
+        acc += b_rows[b];
+        av[b] = b_rows[b];
+        bv[b] = b_cols[b];
+      }
+      fill_segments(count, a_rows, bstart.data(), base.p, ar, s);
+      fill_segments(count, a_rows, av.data(), alim.p, ar, s);
+      fill_segments(count, b_rows, bv.data(), blim.p, br, s);
+    }
+    Csr Av = view(A), Bv = view(B);
+    Av.cols = maxk;
+    validate_csr_dev(Av, A->cols, alim.p, s, "A_stack");
+    validate_csr_dev(Bv, B->cols, blim.p, s, "B_stack");
+    Bv.cols = maxc;
+    CsrOut Cd;
+    Batch bt;
+    bt.b_base = base.p;
+    run_device(Av, Bv, bt, Cd, o, s);
+    export_device(Cd, C);
+  });
+}
+
+void rt_csr_free(rt_csr* C) {
+  if (!C) return;
+  if (C->owner == 1) {
+    if (C->mem == RT_MEM_DEVICE) {
+      dfree(C->row_ptr);
+      dfree(C->col);
+      dfree(C->val);
+    } else {
+      std::free(C->row_ptr);
+      std::free(C->col);
+      std::free(C->val);
+    }
+  }
+  zero_csr(C);
+}
+
+int rt_csr_to_device(const rt_csr* h, rt_csr* d) {
+  zero_csr(d);
+  return guard([&] {
+    RT_CHECK(d != nullptr, RT_E_INVALID, "output is NULL");
+    check_csr_header(h, "input");
+    RT_CHECK(h->mem == RT_MEM_HOST, RT_E_INVALID, "input must be host memory");
+    device();
+    hipStream_t s = lib_stream();
+    CsrOut o;
+    o.rows = h->rows;
+    o.cols = h->cols;
+    o.nnz = h->nnz;
+    o.rp = static_cast<i64*>(dmalloc((h->rows + 1) * sizeof(i64)));
+    o.col = static_cast<i32*>(dmalloc(std::max<i64>(h->nnz, 1) * sizeof(i32)));
+    o.val = static_cast<float*>(dmalloc(std::max<i64>(h->nnz, 1) * sizeof(float)));
+    try {
+      RT_HIP(hipMemcpyAsync(o.rp, h->row_ptr, (h->rows + 1) * sizeof(i64), hipMemcpyHostToDevice, s));
+      if (h->nnz) {
+        RT_HIP(hipMemcpyAsync(o.col, h->col, h->nnz * sizeof(i32), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(o.val, h->val, h->nnz * sizeof(float), hipMemcpyHostToDevice, s));
+      }
+      RT_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      free_out(o);
+      throw;
+    }
+    export_device(o, d);
+  });
+}
+
+int rt_csr_to_host(const rt_csr* d, rt_csr* h) {
+  zero_csr(h);
+  return guard([&] {
+    RT_CHECK(h != nullptr, RT_E_INVALID, "output is NULL");
+    check_csr_header(d, "input");
+    RT_CHECK(d->mem == RT_MEM_DEVICE, RT_E_INVALID, "input must be device memory");
+    device();
+    hipStream_t s = lib_stream();
+    HostCsr o;
+    o.alloc(d->rows, d->cols, d->nnz);
+    RT_HIP(hipMemcpyAsync(o.rp, d->row_ptr, (d->rows + 1) * sizeof(i64), hipMemcpyDeviceToHost, s));
+    if (d->nnz) {
+      RT_HIP(hipMemcpyAsync(o.col, d->col, d->nnz * sizeof(i32), hipMemcpyDeviceToHost, s));
+      RT_HIP(hipMemcpyAsync(o.val, d->val, d->nnz * sizeof(float), hipMemcpyDeviceToHost, s));
+    }
+    RT_HIP(hipStreamSynchronize(s));
+    o.export_to(h);
+  });
+}
+
+int rt_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
+  return guard([&] {
+    RT_CHECK(bytes >= 0 && (bytes == 0 || (dst && src)), RT_E_INVALID, "bad copy arguments");
+    device();
+    if (bytes) {
+      hipStream_t s = lib_stream();
+      RT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, s));
+      RT_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int rt_device_synchronize(void) {
+  return guard([&] {
+    device();
+    RT_HIP(hipDeviceSynchronize());
+  });
+}
+
+int rt_pool_trim(void) {
+  return guard([&] { pool_trim(); });
+}
+
+int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cus) {
+  return guard([&] {
+    const DeviceInfo& d = device();
+    if (name && name_len > 0) {
+      std::strncpy(name, d.name.c_str(), name_len - 1);
+      name[name_len - 1] = '\0';
+    }
+    if (total_mem) *total_mem = (int64_t)d.total_mem;
+    if (cus) *cus = d.cus;
+  });
+}
+
+void rt_prof_reset(void) {
+  try {
+    prof_reset();
+  } catch (...) {
+  }
+}
+int32_t rt_prof_count(void) {
+  try {
+    return prof_count();
+  } catch (...) {
+    return 0;
+  }
+}
+int rt_prof_get(int32_t i, char* name, int64_t name_len, int64_t* launches, double* total_ms) {
+  return guard([&] {
+    std::string n;
+    i64 l = 0;
+    double ms = 0;
+    RT_CHECK(prof_get(i, n, l, ms), RT_E_INVALID, "profile index out of range");
+    if (name && name_len > 0) {
+      std::strncpy(name, n.c_str(), name_len - 1);
+      name[name_len - 1] = '\0';
+    }
+    if (launches) *launches = l;
+    if (total_ms) *total_ms = ms;
+  });
+}
+
+// ---- generators -------------------------------------------------------------
+int rt_gen_uniform_k(int64_t n, int64_t k, uint64_t seed, rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    HostCsr h;
+    gen_uniform_k(n, k, seed, h);
+    h.export_to(out);
+  });
+}
+int rt_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    HostCsr h;
+    gen_rmat(scale, ef, a, b, c, seed, h);
+    h.export_to(out);
+  });
+}
+int rt_gen_stencil27(int64_t n, uint64_t seed, rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    HostCsr h;
+    gen_stencil27(n, seed, h);
+    h.export_to(out);
+  });
+}
+int rt_gen_aggregation(int64_t n, rt_csr* P, rt_csr* R) {
+  zero_csr(P);
+  zero_csr(R);
+  return guard([&] {
+    HostCsr hp, hr;
+    gen_aggregation(n, hp, hr);
+    hp.export_to(P);
+    hr.export_to(R);
+  });
+}
+int rt_gen_lognormal_rows(int64_t rows, int64_t cols, double mu, double sigma, int64_t lo,
+                          int64_t hi, uint64_t seed, uint64_t stream, rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    HostCsr h;
+    gen_lognormal_rows(rows, cols, mu, sigma, lo, hi, seed, stream, h);
+    h.export_to(out);
+  });
+}
+int rt_gen_fixed_normal(int64_t rows, int64_t cols, int64_t k, uint64_t seed, uint64_t stream,
+                        rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    HostCsr h;
+    gen_fixed_normal(rows, cols, k, seed, stream, h);
+    h.export_to(out);
+  });
+}
+int rt_gen_poisson(int64_t n, double lambda, uint64_t seed, uint64_t stream, rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    HostCsr h;
+    gen_poisson(n, lambda, seed, stream, h);
+    h.export_to(out);
+  });
+}
+int rt_gen_batch_poisson(int32_t count, int64_t nmin, int64_t nmax, double lmin, double lmax,
+                         uint64_t seed, int64_t* a_rows, int64_t* b_rows, int64_t* b_cols,
+                         rt_csr* A, rt_csr* B) {
+  zero_csr(A);
+  zero_csr(B);
+  return guard([&] {
+    RT_CHECK(a_rows && b_rows && b_cols && A && B, RT_E_INVALID, "NULL argument");
+    HostCsr ha, hb;
+    gen_batch_poisson(count, nmin, nmax, lmin, lmax, seed, a_rows, b_rows, b_cols, ha, hb);
+    ha.export_to(A);
+    hb.export_to(B);
+  });
+}
+
+// ---- Matrix Market -----------------------------------------------------------
+int rt_mtx_read(const char* path, rt_csr* out) {
+  zero_csr(out);
+  return guard([&] {
+    RT_CHECK(path && out, RT_E_INVALID, "NULL argument");
+    HostCsr h;
+    mtx_read(path, h);
+    h.export_to(out);
+  });
+}
+int rt_mtx_write(const char* path, const rt_csr* C, int32_t ref_compat) {
+  return guard([&] {
+    RT_CHECK(path && C, RT_E_INVALID, "NULL argument");
+    check_csr_header(C, "C");
+    RT_CHECK(C->mem == RT_MEM_HOST, RT_E_INVALID, "rt_mtx_write takes a host CSR");
+    mtx_write(path, *C, ref_compat);
+  });
+}
+
+}  // extern "C"

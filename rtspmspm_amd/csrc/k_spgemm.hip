@@ -1,0 +1,770 @@
+// Exact two-phase SpGEMM for gfx950: row analysis, row binning, LDS-hash and
+// LDS-bitmap symbolic kernels, LDS-hash and dense-LDS-column-tile numeric
+// kernels, exclusive scan, and the host pipeline spgemm_plain().
+//
+// Reference mapping.  The reference computes C = A*B by tracing one ray per
+// nonzero A(i,k) (optixSpMSpM.cu:113-201) whose any-hit program adds
+// A(i,k)*B(k,j) into a dense C for every sphere B(k,j) the ray meets
+// (optixSpMSpM.cu:233-260).  Its GPU comparator is cuSPARSE's two-phase
+// SpGEMM (cuSparse/src/main.cpp:464-539).  Here the "hits" of a ray are the
+// entries of B's row k, read through B's row pointer, and the any-hit
+// accumulate becomes an insert into an LDS table owned by output row i.
+#include <algorithm>
+#include <climits>
+
+#include "device.hpp"
+#include "spgemm.hpp"
+
+namespace rt {
+using namespace dev;
+
+constexpr int NBIN = 6;
+constexpr int TILE_W = 32768;                   // columns per dense numeric tile
+constexpr int WIN_TILES = 32;                   // tiles per symbolic bitmap window
+constexpr i64 WIN_W = (i64)TILE_W * WIN_TILES;  // 2^20 columns per window
+
+struct Thr {
+  i64 t[5];
+};
+
+// ---------------------------------------------------------------------------
+// Row analysis: P_i = sum over A(i,k) of nnz(B(k,:)).  16 lanes per row.
+// totals[0] += sum P_i, totals[1] = max P_i.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_row_products(
+    i64 nrows, const i64* __restrict__ arp, const i32* __restrict__ acol,
+    const i64* __restrict__ brp, const i64* __restrict__ bbase, i64* __restrict__ prod,
+    unsigned long long* __restrict__ totals) {
+  constexpr int G = 16;
+  __shared__ i64 s_scr[4];
+  const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / G;
+  const int l = threadIdx.x & (G - 1);
+  i64 sum = 0;
+  if (row < nrows) {
+    const i64 a0 = arp[row], a1 = arp[row + 1];
+    const i64 bb = BATCH ? bbase[row] : 0;
+    for (i64 e = a0 + l; e < a1; e += G) {
+      const i64 k = (i64)acol[e] + bb;
+      sum += brp[k + 1] - brp[k];
+    }
+  }
+  sum = seg_reduce_sum<G>(sum);
+  if (row < nrows && l == 0) prod[row] = sum;
+  i64 mx = seg_reduce_max<WAVE>(sum);
+  i64 ws = block_reduce_sum<256>(l == 0 ? sum : (i64)0, s_scr);
+  if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(&totals[1], (unsigned long long)mx);
+  if (threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)ws);
+}
+
+// ---------------------------------------------------------------------------
+// Row binning.  key = min(x[i], cap); key 0 -> no bin (empty output row).
+__device__ __forceinline__ int bin_of(i64 k, const Thr& th) {
+  if (k <= 0) return -1;
+#pragma unroll
+  for (int b = 0; b < 5; ++b)
+    if (k <= th.t[b]) return b;
+  return 5;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(i64 n, const i64* __restrict__ x, i64 cap,
+                                                   Thr th, i32* __restrict__ counts) {
+  __shared__ i32 s_cnt[NBIN];
+  if (threadIdx.x < NBIN) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const int b = bin_of(min(x[i], cap), th);
+    if (b >= 0) atomicAdd(&s_cnt[b], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < NBIN && s_cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(i64 n, const i64* __restrict__ x, i64 cap,
+                                                  Thr th, i32* __restrict__ cursor,
+                                                  i32* __restrict__ list) {
+  __shared__ i32 s_cnt[NBIN], s_base[NBIN];
+  if (threadIdx.x < NBIN) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  int b = -1, slot = 0;
+  if (i < n) {
+    b = bin_of(min(x[i], cap), th);
+    if (b >= 0) slot = atomicAdd(&s_cnt[b], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < NBIN && s_cnt[threadIdx.x])
+    s_base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], s_cnt[threadIdx.x]);
+  __syncthreads();
+  if (b >= 0) list[s_base[b] + slot] = (i32)i;
+}
+
+// ---------------------------------------------------------------------------
+// LDS-hash row kernel.  A group of G lanes owns one output row and a table of
+// T slots (linear probing, load factor <= 1/2 by binning).  The products of
+// the row are enumerated flat: a chunk of up to G entries A(i,k) is loaded,
+// their B-row lengths are prefix-summed, and lane l takes products l, l+G, ...
+// of the chunk, locating its A entry by binary search over the prefix sums.
+// Consecutive lanes therefore read consecutive entries of one B row.
+//   NUMERIC = false: count distinct columns -> out_nnz[row].
+//   NUMERIC = true : accumulate values, compact, bitonic-sort by column and
+//                    write the row at crp[row].
+template <int G, int T, int WG, bool NUMERIC, bool BATCH>
+__global__ __launch_bounds__(WG) void k_hash(
+    const i32* __restrict__ list, i32 nlist, const i64* __restrict__ arp,
+    const i32* __restrict__ acol, const float* __restrict__ aval,
+    const i64* __restrict__ brp, const i32* __restrict__ bcol,
+    const float* __restrict__ bval, const i64* __restrict__ bbase,
+    i64* __restrict__ out_nnz, const i64* __restrict__ crp, i32* __restrict__ ccol,
+    float* __restrict__ cval) {
+  static_assert(G <= WAVE || G == WG, "a multi-wave group must be the whole workgroup");
+  constexpr int R = WG / G;
+  constexpr int LT = Log2<T>::value;
+  constexpr int SPL = T / G;
+  __shared__ __align__(16) i32 s_keys[R * T];
+  __shared__ float s_vals[NUMERIC ? R * T : 1];
+  __shared__ i64 s_bs[WG];
+  __shared__ i32 s_ex[WG];
+  __shared__ float s_av[NUMERIC ? WG : 1];
+  __shared__ i32 s_scr[WG / WAVE];
+
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  i32* keys = s_keys + g * T;
+  float* vals = NUMERIC ? s_vals + g * T : nullptr;
+  const int gb = g * G;
+  const i64 ridx = (i64)blockIdx.x * R + g;
+  const bool active = ridx < nlist;
+
+  for (int s = l; s < T; s += G) {
+    keys[s] = EMPTY_KEY;
+    if constexpr (NUMERIC) vals[s] = 0.f;
+  }
+  group_sync<G>();
+
+  const i32 row = active ? list[ridx] : 0;
+  i64 a0 = 0, a1 = 0, bb = 0;
+  if (active) {
+    a0 = arp[row];
+    a1 = arp[row + 1];
+    if constexpr (BATCH) bb = bbase[row];
+  }
+  int mycnt = 0;
+  for (i64 e0 = a0; e0 < a1; e0 += G) {
+    const i64 e = e0 + l;
+    int len = 0;
+    i64 bs = 0;
+    float av = 0.f;
+    if (e < a1) {
+      const i64 k = (i64)acol[e] + bb;
+      bs = brp[k];
+      len = (int)(brp[k + 1] - bs);
+      if constexpr (NUMERIC) av = aval[e];
+    }
+    int inc, total;
+    if constexpr (G <= WAVE) {
+      inc = seg_scan_incl<G>(len);
+      total = __shfl(inc, G - 1, G);
+    } else {
+      inc = block_scan_incl<WG>(len, s_scr, &total);
+    }
+    s_ex[gb + l] = inc - len;
+    s_bs[gb + l] = bs;
+    if constexpr (NUMERIC) s_av[gb + l] = av;
+    group_sync<G>();
+    const int nvalid = (int)min((i64)G, a1 - e0);
+    int lo = 0;
+    for (int p = l; p < total; p += G) {
+      int hi = nvalid - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_ex[gb + mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      const i64 idx = s_bs[gb + lo] + (p - s_ex[gb + lo]);
+      const i32 c = bcol[idx];
+      u32 h = hash_col<LT>(c);
+      if constexpr (NUMERIC) {
+        const float v = s_av[gb + lo] * bval[idx];
+        while (true) {
+          i32 kk = keys[h];
+          if (kk == c) break;
+          if (kk == EMPTY_KEY) {
+            kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+            if (kk == EMPTY_KEY || kk == c) break;
+          }
+          h = (h + 1) & (T - 1);
+        }
+        lds_add_f32(&vals[h], v);
+      } else {
+        while (true) {
+          i32 kk = keys[h];
+          if (kk == c) break;
+          if (kk == EMPTY_KEY) {
+            kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+            if (kk == EMPTY_KEY) { ++mycnt; break; }
+            if (kk == c) break;
+          }
+          h = (h + 1) & (T - 1);
+        }
+      }
+    }
+    group_sync<G>();
+  }
+
+  if constexpr (!NUMERIC) {
+    int tot;
+    if constexpr (G <= WAVE) tot = seg_reduce_sum<G>(mycnt);
+    else tot = block_reduce_sum<WG>(mycnt, s_scr);
+    if (active && l == 0) out_nnz[row] = tot;
+  } else {
+    // Compact the occupied slots (packed column<<32 | value bits) ...
+    u64 mine[SPL];
+    int mc = 0;
+#pragma unroll
+    for (int m = 0; m < SPL; ++m) {
+      const int s = l + m * G;
+      const i32 kk = keys[s];
+      mine[m] = (kk == EMPTY_KEY) ? ~0ull : (((u64)(u32)kk << 32) | __float_as_uint(vals[s]));
+      mc += (kk != EMPTY_KEY);
+    }
+    int inc, N;
+    if constexpr (G <= WAVE) {
+      inc = seg_scan_incl<G>(mc);
+      N = __shfl(inc, G - 1, G);
+    } else {
+      inc = block_scan_incl<WG>(mc, s_scr, &N);
+    }
+    int pos = inc - mc;
+    group_sync<G>();
+    u64* buf = reinterpret_cast<u64*>(keys);  // T/2 packed entries; N <= T/2
+#pragma unroll
+    for (int m = 0; m < SPL; ++m)
+      if (mine[m] != ~0ull) buf[pos++] = mine[m];
+    int S = 1;
+    while (S < N) S <<= 1;
+    for (int q = N + l; q < S; q += G) buf[q] = ~0ull;
+    group_sync<G>();
+    // ... bitonic sort ascending (columns are unique, so the order is total) ...
+    for (int k = 2; k <= S; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int p = l; p < (S >> 1); p += G) {
+          const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+          const int ixj = i + j;
+          const u64 x = buf[i], y = buf[ixj];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            buf[i] = y;
+            buf[ixj] = x;
+          }
+        }
+        group_sync<G>();
+      }
+    }
+    // ... and write the row.
+    if (active) {
+      const i64 off = crp[row];
+      for (int q = l; q < N; q += G) {
+        const u64 v = buf[q];
+        ccol[off + q] = (i32)(v >> 32);
+        cval[off + q] = __uint_as_float((u32)v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-bitmap symbolic kernel for long rows: one workgroup per row, one bit
+// per output column over a window of 2^20 columns (128 KB of LDS); wider
+// matrices take several windows.  Also records the nnz of every 32768-column
+// tile of the row, which positions the dense-tile numeric kernel's output.
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_sym_bitmap(
+    const i32* __restrict__ list, i32 nlist, const i64* __restrict__ arp,
+    const i32* __restrict__ acol, const i64* __restrict__ brp,
+    const i32* __restrict__ bcol, const i64* __restrict__ bbase, i64 ncols, i32 nt,
+    i32* __restrict__ tilecnt, i64* __restrict__ out_nnz) {
+  constexpr int WG = 1024;
+  __shared__ u32 s_bits[WIN_W / 32];
+  __shared__ i64 s_bs[WG];
+  __shared__ i32 s_ex[WG];
+  __shared__ i32 s_scr[WG / WAVE];
+  __shared__ i64 s_scr64[WG / WAVE];
+  const int l = threadIdx.x;
+  const int lane = l & (WAVE - 1), w = l / WAVE;
+  for (i64 it = blockIdx.x; it < nlist; it += gridDim.x) {
+    const i32 row = list[it];
+    const i64 a0 = arp[row], a1 = arp[row + 1];
+    const i64 bb = BATCH ? bbase[row] : 0;
+    i64 total = 0;
+    for (i64 win = 0; win < ncols; win += WIN_W) {
+      const i64 wcols = min(WIN_W, ncols - win);
+      const int nd = (int)((wcols + 31) / 32);
+      for (int d = l; d < nd; d += WG) s_bits[d] = 0;
+      __syncthreads();
+      for (i64 e0 = a0; e0 < a1; e0 += WG) {
+        const i64 e = e0 + l;
+        int len = 0;
+        i64 bs = 0;
+        if (e < a1) {
+          const i64 k = (i64)acol[e] + bb;
+          bs = brp[k];
+          len = (int)(brp[k + 1] - bs);
+        }
+        int tot;
+        const int inc = block_scan_incl<WG>(len, s_scr, &tot);
+        s_ex[l] = inc - len;
+        s_bs[l] = bs;
+        __syncthreads();
+        const int nvalid = (int)min((i64)WG, a1 - e0);
+        int lo = 0;
+        for (int p = l; p < tot; p += WG) {
+          int hi = nvalid - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_ex[mid] <= p) lo = mid; else hi = mid - 1;
+          }
+          const i64 c = (i64)bcol[s_bs[lo] + (p - s_ex[lo])] - win;
+          if (c >= 0 && c < wcols) lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+        }
+        __syncthreads();
+      }
+      const int ntw = (int)((wcols + TILE_W - 1) / TILE_W);
+      i64 wsum = 0;
+      for (int t = w; t < ntw; t += WG / WAVE) {
+        int c = 0;
+        const int dend = min((t + 1) * (TILE_W / 32), nd);
+        for (int d = t * (TILE_W / 32) + lane; d < dend; d += WAVE) c += __popc(s_bits[d]);
+        c = seg_reduce_sum<WAVE>(c);
+        if (lane == 0) tilecnt[it * nt + (win / TILE_W) + t] = c;
+        wsum += c;
+      }
+      total += block_reduce_sum<WG>(lane == 0 ? wsum : (i64)0, s_scr64);
+    }
+    if (l == 0) out_nnz[row] = total;
+    __syncthreads();
+  }
+}
+
+// Per B row, the offset (relative to the row start) of the first entry in
+// each 32768-column tile: slab[k*(nt+1)+t], t = 0..nt.
+__global__ __launch_bounds__(256) void k_slab_ptrs(i64 brows, i32 nt, const i64* __restrict__ brp,
+                                                   const i32* __restrict__ bcol,
+                                                   i32* __restrict__ slab) {
+  const i64 gid = (i64)blockIdx.x * 256 + threadIdx.x;
+  const i64 k = gid / (nt + 1);
+  const int t = (int)(gid - k * (nt + 1));
+  if (k >= brows) return;
+  const i64 r0 = brp[k], r1 = brp[k + 1];
+  i64 res;
+  if (t == 0) {
+    res = 0;
+  } else if (t == nt) {
+    res = r1 - r0;
+  } else {
+    const i64 target = (i64)t * TILE_W;
+    i64 lo = r0, hi = r1;
+    while (lo < hi) {
+      const i64 mid = (lo + hi) >> 1;
+      if (bcol[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    res = lo - r0;
+  }
+  slab[gid] = (i32)res;
+}
+
+// Dense LDS column-tile numeric kernel for long rows.  Work item = (row,
+// tile); the tile's 32768 fp32 accumulators and presence bits live in LDS,
+// the row's products restricted to the tile are read through the B slab
+// pointers, and the tile is emitted in column order by a scan over the
+// presence bits (no sort needed).  LDS is re-zeroed entry by entry as it is
+// emitted so that the next item starts clean.
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_num_tile(
+    const i32* __restrict__ list, i32 nlist, i32 nt, const i32* __restrict__ tilecnt,
+    const i64* __restrict__ nnz_row, i64 th, const i64* __restrict__ arp,
+    const i32* __restrict__ acol, const float* __restrict__ aval,
+    const i64* __restrict__ brp, const i32* __restrict__ bcol,
+    const float* __restrict__ bval, const i64* __restrict__ bbase,
+    const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
+    float* __restrict__ cval) {
+  constexpr int WG = 1024;
+  static_assert(TILE_W / 32 == WG, "one presence word per thread");
+  __shared__ float s_vals[TILE_W];
+  __shared__ u32 s_bits[TILE_W / 32];
+  __shared__ i64 s_bs[WG];
+  __shared__ i32 s_ex[WG];
+  __shared__ float s_av[WG];
+  __shared__ i32 s_scr[WG / WAVE];
+  __shared__ i64 s_off;
+  const int l = threadIdx.x;
+  for (int d = l; d < TILE_W; d += WG) s_vals[d] = 0.f;
+  s_bits[l] = 0;
+  __syncthreads();
+  const i64 nitems = (i64)nlist * nt;
+  for (i64 item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const i64 h = item / nt;
+    const int t = (int)(item - h * nt);
+    const i32 row = list[h];
+    if (nnz_row[row] <= th) continue;
+    if (tilecnt[h * nt + t] == 0) continue;
+    if (l < WAVE) {
+      i64 s = 0;
+      for (int q = l; q < t; q += WAVE) s += tilecnt[h * nt + q];
+      s = seg_reduce_sum<WAVE>(s);
+      if (l == 0) s_off = crp[row] + s;
+    }
+    const i64 a0 = arp[row], a1 = arp[row + 1];
+    const i64 bb = BATCH ? bbase[row] : 0;
+    const i64 cbase = (i64)t * TILE_W;
+    for (i64 e0 = a0; e0 < a1; e0 += WG) {
+      const i64 e = e0 + l;
+      int len = 0;
+      i64 bs = 0;
+      float av = 0.f;
+      if (e < a1) {
+        const i64 k = (i64)acol[e] + bb;
+        const i32* sp = slab + k * (nt + 1);
+        const i32 s0 = sp[t], s1 = sp[t + 1];
+        bs = brp[k] + s0;
+        len = s1 - s0;
+        av = aval[e];
+      }
+      int tot;
+      const int inc = block_scan_incl<WG>(len, s_scr, &tot);
+      s_ex[l] = inc - len;
+      s_bs[l] = bs;
+      s_av[l] = av;
+      __syncthreads();
+      const int nvalid = (int)min((i64)WG, a1 - e0);
+      int lo = 0;
+      for (int p = l; p < tot; p += WG) {
+        int hi = nvalid - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_ex[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        const i64 idx = s_bs[lo] + (p - s_ex[lo]);
+        const int c = (int)(bcol[idx] - cbase);
+        lds_add_f32(&s_vals[c], s_av[lo] * bval[idx]);
+        lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+      }
+      __syncthreads();
+    }
+    const u32 bits = s_bits[l];
+    const int pc = __popc(bits);
+    int tot;
+    const int inc = block_scan_incl<WG>(pc, s_scr, &tot);
+    i64 pos = s_off + (inc - pc);
+    u32 b = bits;
+    while (b) {
+      const int bi = __ffs(b) - 1;
+      b &= b - 1;
+      const int cl = l * 32 + bi;
+      ccol[pos] = (i32)(cbase + cl);
+      cval[pos] = s_vals[cl];
+      s_vals[cl] = 0.f;
+      ++pos;
+    }
+    s_bits[l] = 0;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of n int64 values into out[0..n] (out[n] = total).
+constexpr int SCAN_TILE = 4096;
+
+__global__ __launch_bounds__(1024) void k_scan_reduce(const i64* __restrict__ in, i64 n,
+                                                      i64* __restrict__ bsum) {
+  __shared__ i64 s_scr[16];
+  const i64 base = (i64)blockIdx.x * SCAN_TILE;
+  i64 s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const i64 i = base + threadIdx.x + j * 1024;
+    if (i < n) s += in[i];
+  }
+  s = block_reduce_sum<1024>(s, s_scr);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(i64* __restrict__ bsum, i64 nb,
+                                                   i64* __restrict__ out_total) {
+  __shared__ i64 s_scr[16];
+  i64 carry = 0;
+  for (i64 b0 = 0; b0 < nb; b0 += 1024) {
+    const i64 i = b0 + threadIdx.x;
+    const i64 v = i < nb ? bsum[i] : 0;
+    i64 tot;
+    const i64 inc = block_scan_incl<1024>(v, s_scr, &tot);
+    if (i < nb) bsum[i] = carry + inc - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_apply(const i64* __restrict__ in, i64 n,
+                                                     const i64* __restrict__ bsum,
+                                                     i64* __restrict__ out) {
+  __shared__ i64 s_scr[16];
+  const i64 base = (i64)blockIdx.x * SCAN_TILE + (i64)threadIdx.x * 4;
+  i64 v[4];
+  i64 s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = (base + j < n) ? in[base + j] : 0;
+    s += v[j];
+  }
+  i64 tot;
+  const i64 inc = block_scan_incl<1024>(s, s_scr, &tot);
+  i64 run = bsum[blockIdx.x] + inc - s;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+}
+
+void exclusive_scan_i64(const i64* in, i64* out, i64 n, hipStream_t s) {
+  const i64 nb = std::max<i64>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
+  DBuf<i64> bsum(nb);
+  {
+    ProfScope ps("scan_reduce", s);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, s, in, n, bsum.p);
+  }
+  {
+    ProfScope ps("scan_top", s);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nb, out + n);
+  }
+  if (n > 0) {
+    ProfScope ps("scan_apply", s);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, s, in, n, bsum.p, out);
+  }
+  RT_HIP(hipGetLastError());
+  // bsum is released to the pool here; the pool only hands it out again to
+  // work ordered after this on the same stream or after a synchronisation.
+  RT_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Segment fill: out[r] = values[b] for the segment b containing row r.
+__global__ __launch_bounds__(256) void k_fill_segments(i32 count, const i64* __restrict__ start,
+                                                       const i64* __restrict__ values,
+                                                       i64* __restrict__ out, i64 total) {
+  const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (r >= total) return;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= r) lo = mid; else hi = mid - 1;
+  }
+  out[r] = values[lo];
+}
+
+void fill_segments(i32 count, const i64* seg_rows, const i64* values, i64* out_dev,
+                   i64 total_rows, hipStream_t s) {
+  std::vector<i64> st(count + 1, 0);
+  for (int i = 0; i < count; ++i) st[i + 1] = st[i] + seg_rows[i];
+  DBuf<i64> dst(count + 1), dv(count);
+  RT_HIP(hipMemcpyAsync(dst.p, st.data(), (count + 1) * 8, hipMemcpyHostToDevice, s));
+  RT_HIP(hipMemcpyAsync(dv.p, values, count * 8, hipMemcpyHostToDevice, s));
+  if (total_rows > 0) {
+    ProfScope ps("fill_segments", s);
+    hipLaunchKernelGGL(k_fill_segments, dim3((unsigned)((total_rows + 255) / 256)), dim3(256), 0,
+                       s, count, dst.p, dv.p, out_dev, total_rows);
+  }
+  RT_HIP(hipGetLastError());
+  RT_HIP(hipStreamSynchronize(s));
+}
+
+// Structural validation of a device CSR: row_ptr runs from 0 to nnz and is
+// monotone; columns lie in [0, limit) (limit = row_limit[r] when given) and
+// increase strictly within each row.  flag[0] |= 1 on any violation.
+__global__ __launch_bounds__(256) void k_validate(i64 rows, i64 nnz, const i64* __restrict__ rp,
+                                                  const i32* __restrict__ col, i64 col_limit,
+                                                  const i64* __restrict__ row_limit,
+                                                  int* __restrict__ flag) {
+  constexpr int G = 16;
+  const i64 r = ((i64)blockIdx.x * 256 + threadIdx.x) / G;
+  const int l = threadIdx.x & (G - 1);
+  if (r >= rows) return;
+  const i64 a = rp[r], b = rp[r + 1];
+  bool bad = a > b || a < 0 || b > nnz || (r == 0 && a != 0) || (r == rows - 1 && b != nnz);
+  if (!bad) {
+    const i64 lim = row_limit ? row_limit[r] : col_limit;
+    for (i64 e = a + l; e < b; e += G) {
+      const i32 c = col[e];
+      if (c < 0 || c >= lim || (e > a && col[e - 1] >= c)) bad = true;
+    }
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
+                      const char* what) {
+  DBuf<int> flag(1);
+  RT_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  if (M.rows > 0) {
+    ProfScope ps("validate", s);
+    hipLaunchKernelGGL(k_validate, dim3((unsigned)((M.rows * 16 + 255) / 256)), dim3(256), 0, s,
+                       M.rows, M.nnz, M.rp, M.col, col_limit, row_limit, flag.p);
+    RT_HIP(hipGetLastError());
+  } else {
+    RT_CHECK(M.rows == 0, RT_E_INVALID, "negative rows");
+  }
+  const int f = read_scalar(flag.p, s);
+  RT_CHECK(f == 0, RT_E_INVALID,
+           std::string(what) + ": malformed CSR (row_ptr must run monotonically from 0 to nnz; "
+           "columns must be in range, sorted and unique within each row)");
+}
+
+// ---------------------------------------------------------------------------
+// Host pipeline.
+namespace {
+
+template <int G, int T, int WG, bool NUM, bool BATCH>
+void launch_hash(const char* name, const i32* list, i32 n, const Csr& A, const Csr& B,
+                 const i64* bbase, i64* nnz, const CsrOut* C, hipStream_t s) {
+  if (n <= 0) return;
+  constexpr int R = WG / G;
+  const dim3 grid((unsigned)((n + R - 1) / R));
+  ProfScope ps(name, s);
+  hipLaunchKernelGGL((k_hash<G, T, WG, NUM, BATCH>), grid, dim3(WG), 0, s, list, n, A.rp, A.col,
+                     A.val, B.rp, B.col, B.val, bbase, nnz, C ? C->rp : nullptr,
+                     C ? C->col : nullptr, C ? C->val : nullptr);
+  RT_HIP(hipGetLastError());
+}
+
+template <bool NUM, bool BATCH>
+void launch_hash_bins(const i32* list, const i32* cnt, const i32* off, const Csr& A,
+                      const Csr& B, const i64* bbase, i64* nnz, const CsrOut* C,
+                      hipStream_t s) {
+  launch_hash<8, 32, 256, NUM, BATCH>(NUM ? "num_hash_t32" : "sym_hash_t32", list + off[0],
+                                      cnt[0], A, B, bbase, nnz, C, s);
+  launch_hash<16, 128, 256, NUM, BATCH>(NUM ? "num_hash_t128" : "sym_hash_t128",
+                                        list + off[1], cnt[1], A, B, bbase, nnz, C, s);
+  launch_hash<64, 512, 256, NUM, BATCH>(NUM ? "num_hash_t512" : "sym_hash_t512",
+                                        list + off[2], cnt[2], A, B, bbase, nnz, C, s);
+  launch_hash<256, 2048, 256, NUM, BATCH>(NUM ? "num_hash_t2048" : "sym_hash_t2048",
+                                          list + off[3], cnt[3], A, B, bbase, nnz, C, s);
+  launch_hash<1024, 8192, 1024, NUM, BATCH>(NUM ? "num_hash_t8192" : "sym_hash_t8192",
+                                            list + off[4], cnt[4], A, B, bbase, nnz, C, s);
+}
+
+// Bin the rows by min(x, cap); returns host counts and offsets of each bin
+// inside `list`.
+void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int cnt[NBIN],
+              int off[NBIN + 1], hipStream_t s) {
+  Thr th;
+  for (int b = 0; b < 5; ++b) th.t[b] = th_in[b];
+  DBuf<i32> dcnt(NBIN);
+  RT_HIP(hipMemsetAsync(dcnt.p, 0, NBIN * sizeof(i32), s));
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (n > 0) {
+    ProfScope ps("bin_count", s);
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, dcnt.p);
+  }
+  RT_HIP(hipGetLastError());
+  RT_HIP(hipMemcpyAsync(cnt, dcnt.p, NBIN * sizeof(i32), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipStreamSynchronize(s));
+  off[0] = 0;
+  for (int b = 0; b < NBIN; ++b) off[b + 1] = off[b] + cnt[b];
+  RT_HIP(hipMemcpyAsync(dcnt.p, off, NBIN * sizeof(i32), hipMemcpyHostToDevice, s));
+  if (n > 0) {
+    ProfScope ps("bin_fill", s);
+    hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, dcnt.p, list);
+  }
+  RT_HIP(hipGetLastError());
+  RT_HIP(hipStreamSynchronize(s));
+}
+
+template <bool BATCH>
+void plain_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, const Tuning& tu,
+                hipStream_t s, RunStats* st) {
+  const i64 m = A.rows;
+  RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
+  C.rows = m;
+  C.cols = B.cols;
+  DBuf<i64> prod(std::max<i64>(m, 1)), nnz(std::max<i64>(m, 1));
+  DBuf<unsigned long long> tot(2);
+  RT_HIP(hipMemsetAsync(tot.p, 0, 2 * sizeof(unsigned long long), s));
+  if (m > 0) {
+    ProfScope ps("row_products", s);
+    hipLaunchKernelGGL(k_row_products<BATCH>, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0,
+                       s, m, A.rp, A.col, B.rp, bbase, prod.p, tot.p);
+  }
+  RT_HIP(hipGetLastError());
+  unsigned long long htot[2];
+  RT_HIP(hipMemcpyAsync(htot, tot.p, sizeof(htot), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipStreamSynchronize(s));
+  if (st) {
+    st->products = (i64)htot[0];
+    st->max_row_products = (i64)htot[1];
+  }
+  RT_HIP(hipMemsetAsync(nnz.p, 0, std::max<i64>(m, 1) * sizeof(i64), s));
+
+  // ---- symbolic ----
+  DBuf<i32> slist(std::max<i64>(m, 1));
+  int scnt[NBIN], soff[NBIN + 1];
+  bin_rows(prod.p, m, B.cols, tu.sym, slist.p, scnt, soff, s);
+  launch_hash_bins<false, BATCH>(slist.p, scnt, soff, A, B, bbase, nnz.p, nullptr, s);
+  const i32 nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
+  DBuf<i32> tilecnt;
+  if (scnt[5] > 0) {
+    tilecnt.alloc((size_t)scnt[5] * nt);
+    const unsigned grid = (unsigned)std::min<i64>(scnt[5], (i64)device().cus * 8);
+    ProfScope ps("sym_bitmap", s);
+    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, slist.p + soff[5],
+                       scnt[5], A.rp, A.col, B.rp, B.col, bbase, B.cols, nt, tilecnt.p, nnz.p);
+    RT_HIP(hipGetLastError());
+  }
+
+  // ---- row pointers ----
+  C.rp = static_cast<i64*>(dmalloc((m + 1) * sizeof(i64)));
+  exclusive_scan_i64(nnz.p, C.rp, m, s);
+  C.nnz = read_scalar(C.rp + m, s);
+  C.col = static_cast<i32*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(i32)));
+  C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
+
+  // ---- numeric ----
+  DBuf<i32> nlist(std::max<i64>(m, 1));
+  int ncnt[NBIN], noff[NBIN + 1];
+  bin_rows(nnz.p, m, LLONG_MAX, tu.num, nlist.p, ncnt, noff, s);
+  if (ncnt[5] > 0) {
+    RT_CHECK(scnt[5] > 0, RT_E_INVALID, "tuning: numeric tile rows without symbolic tile counts");
+    DBuf<i32> slab((size_t)B.rows * (nt + 1));
+    {
+      const i64 nthreads = B.rows * (i64)(nt + 1);
+      ProfScope ps("slab_ptrs", s);
+      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
+                         B.rows, nt, B.rp, B.col, slab.p);
+    }
+    const i64 nitems = (i64)scnt[5] * nt;
+    const unsigned grid = (unsigned)std::min<i64>(nitems, (i64)device().cus * 4);
+    {
+      ProfScope ps("num_tile", s);
+      hipLaunchKernelGGL(k_num_tile<BATCH>, dim3(grid), dim3(1024), 0, s, slist.p + soff[5],
+                         scnt[5], nt, tilecnt.p, nnz.p, tu.num[4], A.rp, A.col, A.val, B.rp,
+                         B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
+    }
+    RT_HIP(hipGetLastError());
+    launch_hash_bins<true, BATCH>(nlist.p, ncnt, noff, A, B, bbase, nullptr, &C, s);
+    RT_HIP(hipStreamSynchronize(s));
+  } else {
+    launch_hash_bins<true, BATCH>(nlist.p, ncnt, noff, A, B, bbase, nullptr, &C, s);
+    RT_HIP(hipStreamSynchronize(s));
+  }
+}
+
+}  // namespace
+
+void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                  const Tuning& tune, hipStream_t s, RunStats* st) {
+  if (batch.b_base)
+    plain_impl<true>(A, B, batch.b_base, C, tune, s, st);
+  else
+    plain_impl<false>(A, B, nullptr, C, tune, s, st);
+}
+
+}  // namespace rt

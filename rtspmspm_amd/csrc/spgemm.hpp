@@ -1,0 +1,78 @@
+// Internal interface between the C ABI (capi.cpp) and the device pipelines.
+#pragma once
+#include "common.hpp"
+
+namespace rt {
+
+// Device CSR operand (borrowed).
+struct Csr {
+  i64 rows = 0, cols = 0, nnz = 0;
+  const i64* rp = nullptr;
+  const i32* col = nullptr;
+  const float* val = nullptr;
+};
+
+// Device CSR result; arrays come from the pool (dmalloc) and are released
+// with dfree (rt_csr_free does that for library-owned device results).
+struct CsrOut {
+  i64 rows = 0, cols = 0, nnz = 0;
+  i64* rp = nullptr;
+  i32* col = nullptr;
+  float* val = nullptr;
+};
+
+// Batched problems: A row r belongs to the pair whose B rows start at
+// b_base[r] in the stacked B.  nullptr for a single product.
+struct Batch {
+  const i64* b_base = nullptr;  // per A row
+};
+
+// Row-bin thresholds of the two-phase method (inclusive upper bounds).
+// Symbolic bins use U = min(products, cols); numeric bins use exact nnz.
+// Bins 0..4 are LDS-hash kernels with tables of 32/128/512/2048/8192 slots
+// (symbolic: keys; numeric: keys + values); bin 5 (above sym[4] / num[4]) is
+// the LDS-bitmap symbolic kernel / the dense LDS column-tile numeric kernel.
+// Invariant: sym[4] <= num[4] so every numeric bin-5 row has tile counts.
+struct Tuning {
+  i64 sym[5] = {16, 64, 256, 1024, 4096};
+  i64 num[5] = {16, 64, 256, 1024, 4096};
+};
+
+struct RunStats {
+  i64 products = 0;
+  i64 max_row_products = 0;
+  i64 overflow_rows = 0;
+  i64 predicted_rows = 0;
+};
+
+// Exact two-phase product.  C.rp/col/val allocated here.
+void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                  const Tuning& tune, hipStream_t s, RunStats* st);
+
+// Faithful one-work-item-per-A-nonzero launch (reference raygen + any-hit).
+void spgemm_ray(const Csr& A, const Csr& B, CsrOut& C, int ablation, int output,
+                hipStream_t s, RunStats* st);
+
+// Learned row sizing: predicted capacities, exact rerun of overflow rows.
+void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                    const Tuning& tune, hipStream_t s, RunStats* st);
+
+// Per-row features (n x RT_NFEAT, device) and optionally exact nnz.
+void row_features(const Csr& A, const Csr& B, const Batch& batch, float* feat,
+                  const i64* prod, hipStream_t s);
+
+// Device utilities.
+void exclusive_scan_i64(const i64* in, i64* out /* n+1 */, i64 n, hipStream_t s);
+// out[r] = values[b] for every row r of segment b (segments of seg_rows[b]
+// consecutive rows); used for the per-row B base and column limits of batches.
+void fill_segments(i32 count, const i64* seg_rows, const i64* values, i64* out_dev,
+                   i64 total_rows, hipStream_t s);
+
+// Predictor state (weights live on the host; uploaded per call).
+struct Mlp;
+const float* mlp_params_host();
+float mlp_margin();
+void mlp_predict_dev(const float* params_dev, const float* feat, i64 n, float* out,
+                     hipStream_t s);
+
+}  // namespace rt

@@ -1,0 +1,157 @@
+"""GPU parity of the HIP SpGEMM paths against the CPU oracle and the golden
+fixtures.  Structure must match bit-exactly; values must be within
+1e-5 * sum|a*b| of the FP64 oracle, entry by entry (REL_TOL in helpers)."""
+import numpy as np
+import pytest
+
+import oracle
+import rtspmspm_amd as rt
+from tests.helpers import (FIXTURES, check_against_golden, check_parity, csr_from_coo, golden,
+                           load_fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+def run(A, B, **kw):
+    C, info = rt.spmspm(A, B, **kw)
+    return C, info
+
+
+# ---- 4,096 x 4,096, 8 uniform columns per row, seed 0 ----------
+@pytest.fixture(scope="module")
+def cfg1():
+    A = rt.gen_uniform_k(4096, 8, 0)
+    return A, oracle.spgemm_f64(A, A)
+
+
+def test_cfg1_plain(cfg1):
+    A, ref = cfg1
+    C, info = run(A, A, mode="plain")
+    assert info["products"] == 4096 * 8 * 8
+    err = check_parity(C, ref)
+    print(f"cfg1 plain: nnz {C.nnz}, max err/sum|ab| {err:.2e}")
+
+
+@pytest.mark.parametrize("out", ["dense", "slot"])
+def test_cfg1_ray(cfg1, out):
+    A, ref = cfg1
+    C, _ = run(A, A, mode="ray", ray_output=out)
+    err = check_parity(C, ref)
+    print(f"cfg1 ray/{out}: max err/sum|ab| {err:.2e}")
+
+
+@pytest.mark.parametrize("abl", ["no_atomic", "traverse", "no_hit", "empty"])
+def test_ray_ablations_run(cfg1, abl):
+    A, _ = cfg1
+    C, info = run(A, A, mode="ray", ablation=abl, ray_output="dense")
+    assert C.shape == (4096, 4096) and info["ms"] > 0
+
+
+# ---- the nine shipped fixtures, A*A ---------------------------------------
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_plain(name):
+    A = load_fixture(name)
+    C, _ = run(A, A, mode="plain")
+    check_against_golden(C, golden()[name])
+    check_parity(C, oracle.spgemm_f64(A, A))
+    if golden()[name]["all_ones"] and golden()[name]["max_row_nnz"] < 2**24:
+        assert np.array_equal(C.val, oracle.spgemm_f32(A, A).val)  # integer sums: exact
+
+
+@pytest.mark.parametrize("name", ["wiki-Vote", "mario002", "web-Google", "ca-CondMat"])
+def test_fixture_ray(name):
+    A = load_fixture(name)
+    C, _ = run(A, A, mode="ray")
+    check_against_golden(C, golden()[name])
+    check_parity(C, oracle.spgemm_f64(A, A))
+
+
+# ---- structure edge cases ---------------------------------------------------
+def test_cancellation_zero_is_kept():
+    A = csr_from_coo(1, 2, [0, 0], [0, 1], [1.0, 1.0])
+    B = csr_from_coo(2, 1, [0, 1], [0, 0], [0.75, -0.75])
+    for mode in ("plain", "ray"):
+        C, _ = run(A, B, mode=mode)
+        assert C.row_ptr.tolist() == [0, 1] and C.col.tolist() == [0] and C.val.tolist() == [0.0]
+
+
+def test_empty_and_rectangular():
+    A = csr_from_coo(4, 3, [0, 2, 2], [1, 0, 2], [2.0, 3.0, 4.0])
+    B = csr_from_coo(3, 5, [0, 1, 1], [4, 0, 3], [1.0, 5.0, 6.0])
+    for mode in ("plain", "ray"):
+        C, _ = run(A, B, mode=mode)
+        check_parity(C, oracle.spgemm_f64(A, B))
+    E = csr_from_coo(3, 5, [], [], [])
+    C, _ = run(A, E)
+    assert C.nnz == 0 and C.row_ptr.tolist() == [0] * 5
+    Z = csr_from_coo(0, 3, [], [], [])
+    C, _ = run(Z, csr_from_coo(3, 2, [0], [1], [1.0]))
+    assert C.shape == (0, 2) and C.nnz == 0
+    one = csr_from_coo(1, 1, [0], [0], [3.0])
+    C, _ = run(one, one)
+    assert C.val.tolist() == [9.0]
+
+
+def test_shape_mismatch_is_an_error():
+    A = csr_from_coo(2, 3, [0], [0], [1.0])
+    with pytest.raises(rt.RtError) as e:
+        run(A, A)
+    assert e.value.code == rt.RT_E_SHAPE
+
+
+# ---- every bin, including the LDS-bitmap and dense-tile kernels --------------
+@pytest.mark.parametrize("n,k,seed", [(3000, 40, 1), (20000, 100, 2), (60000, 160, 3)])
+def test_long_rows_uniform(n, k, seed):
+    """k^2 products per row: 1,600 (hash bins) up to 25,600 (bitmap + tiles)."""
+    A = rt.gen_uniform_k(n, k, seed)
+    C, info = run(A, A)
+    assert info["products"] == n * k * k
+    check_parity(C, oracle.spgemm_f64(A, A))
+
+
+def test_mixed_row_lengths_rmat():
+    A = rt.gen_rmat(14, 16, seed=5)
+    C, _ = run(A, A)
+    err = check_parity(C, oracle.spgemm_f64(A, A))
+    print(f"rmat14: nnz {C.nnz}, max err/sum|ab| {err:.2e}")
+
+
+def test_wide_output_multiple_bitmap_windows():
+    """B with 1.5M columns: the symbolic bitmap takes two 2^20-column windows."""
+    A = rt.gen_fixed_normal(64, 3000, 300, 6)
+    B = rt.gen_fixed_normal(3000, 1_500_000, 40, 7)
+    C, _ = run(A, B)
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
+def test_structure_is_deterministic():
+    A = rt.gen_rmat(13, 16, seed=8)
+    C1, _ = run(A, A)
+    C2, _ = run(A, A)
+    assert np.array_equal(C1.row_ptr, C2.row_ptr) and np.array_equal(C1.col, C2.col)
+
+
+def test_device_resident_operands():
+    A = rt.gen_uniform_k(5000, 12, 9)
+    dA = rt.to_device(A)
+    dC, info = rt.spmspm(dA, dA)
+    assert isinstance(dC, rt.DeviceCSR)
+    check_parity(dC.to_host(), oracle.spgemm_f64(A, A))
+
+
+def test_batched_matches_each_pair():
+    ar, br, bc, A, B = rt.gen_batch_poisson(24, 16, 3000, 4.0, 32.0, 4)
+    dA, dB = rt.to_device(A), rt.to_device(B)
+    dC, info = rt.spmspm_batched(ar, br, bc, dA, dB)
+    C = dC.to_host()
+    a0 = b0 = 0
+    for p in range(len(ar)):
+        Ap = rt.CSR((int(ar[p]), int(br[p])), A.row_ptr[a0:a0 + ar[p] + 1] - A.row_ptr[a0],
+                    A.col[A.row_ptr[a0]:A.row_ptr[a0 + ar[p]]], A.val[A.row_ptr[a0]:A.row_ptr[a0 + ar[p]]])
+        Bp = rt.CSR((int(br[p]), int(bc[p])), B.row_ptr[b0:b0 + br[p] + 1] - B.row_ptr[b0],
+                    B.col[B.row_ptr[b0]:B.row_ptr[b0 + br[p]]], B.val[B.row_ptr[b0]:B.row_ptr[b0 + br[p]]])
+        s, e = C.row_ptr[a0], C.row_ptr[a0 + ar[p]]
+        Cp = rt.CSR((int(ar[p]), int(bc[p])), C.row_ptr[a0:a0 + ar[p] + 1] - s, C.col[s:e], C.val[s:e])
+        check_parity(Cp, oracle.spgemm_f64(Ap, Bp))
+        a0 += int(ar[p])
+        b0 += int(br[p])
