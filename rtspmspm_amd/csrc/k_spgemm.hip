@@ -106,18 +106,38 @@ __global__ __launch_bounds__(256) void k_bin_fill(i64 n, const i64* __restrict__
 // their B-row lengths are prefix-summed, and lane l takes products l, l+G, ...
 // of the chunk, locating its A entry by binary search over the prefix sums.
 // Consecutive lanes therefore read consecutive entries of one B row.
-//   NUMERIC = false: count distinct columns -> out_nnz[row].
-//   NUMERIC = true : accumulate values, compact, bitonic-sort by column and
-//                    write the row at crp[row].
-template <int G, int T, int WG, bool NUMERIC, bool BATCH>
-__global__ __launch_bounds__(WG) void k_hash(
-    const i32* __restrict__ list, i32 nlist, const i64* __restrict__ arp,
-    const i32* __restrict__ acol, const float* __restrict__ aval,
-    const i64* __restrict__ brp, const i32* __restrict__ bcol,
-    const float* __restrict__ bval, const i64* __restrict__ bbase,
-    i64* __restrict__ out_nnz, const i64* __restrict__ crp, i32* __restrict__ ccol,
-    float* __restrict__ cval) {
+//   MODE_SYM  : count distinct columns -> a.nnz[row].
+//   MODE_NUM  : accumulate values, compact, bitonic-sort by column and write
+//               the row at a.crp[row] of (a.ccol, a.cval).
+//   MODE_LEARN: as MODE_NUM, but the row's size is a predicted capacity
+//               a.cap[row] (the table has T >= 2*cap slots) and the row is
+//               written at a.crp[row] of a temporary buffer; a row whose
+//               distinct columns exceed the capacity is abandoned and flagged
+//               in a.ovf[row] for the exact rerun.  a.nnz[row] gets the count.
+constexpr int MODE_SYM = 0, MODE_NUM = 1, MODE_LEARN = 2;
+
+struct HashArgs {
+  const i32* list;
+  i32 nlist;
+  const i64* arp;
+  const i32* acol;
+  const float* aval;
+  const i64* brp;
+  const i32* bcol;
+  const float* bval;
+  const i64* bbase;
+  i64* nnz;
+  const i64* crp;
+  i32* ccol;
+  float* cval;
+  const i64* cap;
+  i32* ovf;
+};
+
+template <int G, int T, int WG, int MODE, bool BATCH>
+__global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   static_assert(G <= WAVE || G == WG, "a multi-wave group must be the whole workgroup");
+  constexpr bool NUMERIC = MODE != MODE_SYM;
   constexpr int R = WG / G;
   constexpr int LT = Log2<T>::value;
   constexpr int SPL = T / G;
@@ -127,6 +147,16 @@ __global__ __launch_bounds__(WG) void k_hash(
   __shared__ i32 s_ex[WG];
   __shared__ float s_av[NUMERIC ? WG : 1];
   __shared__ i32 s_scr[WG / WAVE];
+  __shared__ i32 s_cnt[MODE == MODE_LEARN ? R : 1];
+  __shared__ i32 s_ovf[MODE == MODE_LEARN ? R : 1];
+  const i32* __restrict__ list = a.list;
+  const i32 nlist = a.nlist;
+  const i64* __restrict__ arp = a.arp;
+  const i32* __restrict__ acol = a.acol;
+  const float* __restrict__ aval = a.aval;
+  const i64* __restrict__ brp = a.brp;
+  const i32* __restrict__ bcol = a.bcol;
+  const float* __restrict__ bval = a.bval;
 
   const int g = threadIdx.x / G;
   const int l = threadIdx.x & (G - 1);
@@ -140,14 +170,21 @@ __global__ __launch_bounds__(WG) void k_hash(
     keys[s] = EMPTY_KEY;
     if constexpr (NUMERIC) vals[s] = 0.f;
   }
+  if constexpr (MODE == MODE_LEARN) {
+    if (l == 0) {
+      s_cnt[g] = 0;
+      s_ovf[g] = 0;
+    }
+  }
   group_sync<G>();
 
   const i32 row = active ? list[ridx] : 0;
-  i64 a0 = 0, a1 = 0, bb = 0;
+  i64 a0 = 0, a1 = 0, bb = 0, cap = 0;
   if (active) {
     a0 = arp[row];
     a1 = arp[row + 1];
-    if constexpr (BATCH) bb = bbase[row];
+    if constexpr (BATCH) bb = a.bbase[row];
+    if constexpr (MODE == MODE_LEARN) cap = a.cap[row];
   }
   int mycnt = 0;
   for (i64 e0 = a0; e0 < a1; e0 += G) {
@@ -183,7 +220,29 @@ __global__ __launch_bounds__(WG) void k_hash(
       const i64 idx = s_bs[gb + lo] + (p - s_ex[gb + lo]);
       const i32 c = bcol[idx];
       u32 h = hash_col<LT>(c);
-      if constexpr (NUMERIC) {
+      if constexpr (MODE == MODE_LEARN) {
+        // Capacity-limited insert: the row is abandoned (and flagged) as soon
+        // as it holds more distinct columns than its predicted capacity; the
+        // probe count is bounded by T as well, so no lane can spin.
+        if (s_ovf[g]) continue;
+        const float v = s_av[gb + lo] * bval[idx];
+        bool ok = false;
+        for (int probe = 0; probe < T; ++probe) {
+          i32 kk = keys[h];
+          if (kk == c) { ok = true; break; }
+          if (kk == EMPTY_KEY) {
+            kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+            if (kk == EMPTY_KEY) {
+              if (atomicAdd(&s_cnt[g], 1) >= cap) s_ovf[g] = 1;
+              ok = true;
+              break;
+            }
+            if (kk == c) { ok = true; break; }
+          }
+          h = (h + 1) & (T - 1);
+        }
+        if (ok) lds_add_f32(&vals[h], v); else s_ovf[g] = 1;
+      } else if constexpr (NUMERIC) {
         const float v = s_av[gb + lo] * bval[idx];
         while (true) {
           i32 kk = keys[h];
@@ -215,8 +274,18 @@ __global__ __launch_bounds__(WG) void k_hash(
     int tot;
     if constexpr (G <= WAVE) tot = seg_reduce_sum<G>(mycnt);
     else tot = block_reduce_sum<WG>(mycnt, s_scr);
-    if (active && l == 0) out_nnz[row] = tot;
+    if (active && l == 0) a.nnz[row] = tot;
   } else {
+    if constexpr (MODE == MODE_LEARN) {
+      // s_ovf is uniform over the group (and over the workgroup when G == WG).
+      if (s_ovf[g]) {
+        if (active && l == 0) {
+          a.ovf[row] = 1;
+          a.nnz[row] = 0;
+        }
+        return;
+      }
+    }
     // Compact the occupied slots (packed column<<32 | value bits) ...
     u64 mine[SPL];
     int mc = 0;
@@ -262,11 +331,14 @@ __global__ __launch_bounds__(WG) void k_hash(
     }
     // ... and write the row.
     if (active) {
-      const i64 off = crp[row];
+      const i64 off = a.crp[row];
       for (int q = l; q < N; q += G) {
         const u64 v = buf[q];
-        ccol[off + q] = (i32)(v >> 32);
-        cval[off + q] = __uint_as_float((u32)v);
+        a.ccol[off + q] = (i32)(v >> 32);
+        a.cval[off + q] = __uint_as_float((u32)v);
+      }
+      if constexpr (MODE == MODE_LEARN) {
+        if (l == 0) a.nnz[row] = N;
       }
     }
   }
@@ -620,42 +692,138 @@ void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStre
 }
 
 // ---------------------------------------------------------------------------
-// Host pipeline.
+// Host pipeline pieces (shared by the plain and learned methods).
 namespace {
 
-template <int G, int T, int WG, bool NUM, bool BATCH>
-void launch_hash(const char* name, const i32* list, i32 n, const Csr& A, const Csr& B,
-                 const i64* bbase, i64* nnz, const CsrOut* C, hipStream_t s) {
+template <int G, int T, int WG, int MODE, bool BATCH>
+void launch_hash(const char* name, HashArgs a, const i32* list, i32 n, hipStream_t s) {
   if (n <= 0) return;
   constexpr int R = WG / G;
+  a.list = list;
+  a.nlist = n;
   const dim3 grid((unsigned)((n + R - 1) / R));
   ProfScope ps(name, s);
-  hipLaunchKernelGGL((k_hash<G, T, WG, NUM, BATCH>), grid, dim3(WG), 0, s, list, n, A.rp, A.col,
-                     A.val, B.rp, B.col, B.val, bbase, nnz, C ? C->rp : nullptr,
-                     C ? C->col : nullptr, C ? C->val : nullptr);
+  hipLaunchKernelGGL((k_hash<G, T, WG, MODE, BATCH>), grid, dim3(WG), 0, s, a);
   RT_HIP(hipGetLastError());
 }
 
-template <bool NUM, bool BATCH>
-void launch_hash_bins(const i32* list, const i32* cnt, const i32* off, const Csr& A,
-                      const Csr& B, const i64* bbase, i64* nnz, const CsrOut* C,
+template <int MODE, bool BATCH>
+void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const int* off,
                       hipStream_t s) {
-  launch_hash<8, 32, 256, NUM, BATCH>(NUM ? "num_hash_t32" : "sym_hash_t32", list + off[0],
-                                      cnt[0], A, B, bbase, nnz, C, s);
-  launch_hash<16, 128, 256, NUM, BATCH>(NUM ? "num_hash_t128" : "sym_hash_t128",
-                                        list + off[1], cnt[1], A, B, bbase, nnz, C, s);
-  launch_hash<64, 512, 256, NUM, BATCH>(NUM ? "num_hash_t512" : "sym_hash_t512",
-                                        list + off[2], cnt[2], A, B, bbase, nnz, C, s);
-  launch_hash<256, 2048, 256, NUM, BATCH>(NUM ? "num_hash_t2048" : "sym_hash_t2048",
-                                          list + off[3], cnt[3], A, B, bbase, nnz, C, s);
-  launch_hash<1024, 8192, 1024, NUM, BATCH>(NUM ? "num_hash_t8192" : "sym_hash_t8192",
-                                            list + off[4], cnt[4], A, B, bbase, nnz, C, s);
+  static const char* names[3][5] = {
+      {"sym_hash_t32", "sym_hash_t128", "sym_hash_t512", "sym_hash_t2048", "sym_hash_t8192"},
+      {"num_hash_t32", "num_hash_t128", "num_hash_t512", "num_hash_t2048", "num_hash_t8192"},
+      {"learn_hash_t32", "learn_hash_t128", "learn_hash_t512", "learn_hash_t2048",
+       "learn_hash_t8192"}};
+  launch_hash<8, 32, 256, MODE, BATCH>(names[MODE][0], a, list + off[0], cnt[0], s);
+  launch_hash<16, 128, 256, MODE, BATCH>(names[MODE][1], a, list + off[1], cnt[1], s);
+  launch_hash<64, 512, 256, MODE, BATCH>(names[MODE][2], a, list + off[2], cnt[2], s);
+  launch_hash<256, 2048, 256, MODE, BATCH>(names[MODE][3], a, list + off[3], cnt[3], s);
+  launch_hash<1024, 8192, 1024, MODE, BATCH>(names[MODE][4], a, list + off[4], cnt[4], s);
 }
 
-// Bin the rows by min(x, cap); returns host counts and offsets of each bin
-// inside `list`.
-void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int cnt[NBIN],
-              int off[NBIN + 1], hipStream_t s) {
+HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
+  HashArgs a{};
+  a.arp = A.rp;
+  a.acol = A.col;
+  a.aval = A.val;
+  a.brp = B.rp;
+  a.bcol = B.col;
+  a.bval = B.val;
+  a.bbase = bbase;
+  return a;
+}
+
+template <bool BATCH>
+void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key, i64* nnz,
+                   const Tuning& tu, hipStream_t s, HeavyRows& heavy) {
+  const i64 m = A.rows;
+  heavy.list.alloc(std::max<i64>(m, 1));
+  int cnt[NBIN], off[NBIN + 1];
+  bin_rows(key, m, B.cols, tu.sym, heavy.list.p, cnt, off, s);
+  HashArgs a = make_args(A, B, bbase);
+  a.nnz = nnz;
+  launch_hash_bins<MODE_SYM, BATCH>(a, heavy.list.p, cnt, off, s);
+  heavy.off = off[5];
+  heavy.n = cnt[5];
+  heavy.nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
+  if (heavy.n > 0) {
+    heavy.tilecnt.alloc((size_t)heavy.n * heavy.nt);
+    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 8);
+    ProfScope ps("sym_bitmap", s);
+    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, heavy.list.p + heavy.off,
+                       heavy.n, A.rp, A.col, B.rp, B.col, bbase, B.cols, heavy.nt,
+                       heavy.tilecnt.p, nnz);
+    RT_HIP(hipGetLastError());
+  }
+}
+
+template <bool BATCH>
+void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                  const HeavyRows& heavy, CsrOut& C, const Tuning& tu, hipStream_t s) {
+  const i64 m = A.rows;
+  DBuf<i32> list(std::max<i64>(m, 1));
+  int cnt[NBIN], off[NBIN + 1];
+  bin_rows(key, m, LLONG_MAX, tu.num, list.p, cnt, off, s);
+  if (cnt[5] > 0) {
+    RT_CHECK(heavy.n > 0, RT_E_INVALID, "tuning: numeric tile rows without symbolic tile counts");
+    DBuf<i32> slab((size_t)B.rows * (heavy.nt + 1));
+    {
+      const i64 nthreads = B.rows * (i64)(heavy.nt + 1);
+      ProfScope ps("slab_ptrs", s);
+      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
+                         B.rows, heavy.nt, B.rp, B.col, slab.p);
+      RT_HIP(hipGetLastError());
+    }
+    const i64 nitems = (i64)heavy.n * heavy.nt;
+    const unsigned grid = (unsigned)std::min<i64>(nitems, (i64)device().cus * 4);
+    {
+      ProfScope ps("num_tile", s);
+      hipLaunchKernelGGL(k_num_tile<BATCH>, dim3(grid), dim3(1024), 0, s, heavy.list.p + heavy.off,
+                         heavy.n, heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val,
+                         B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
+      RT_HIP(hipGetLastError());
+    }
+    HashArgs a = make_args(A, B, bbase);
+    a.crp = C.rp;
+    a.ccol = C.col;
+    a.cval = C.val;
+    launch_hash_bins<MODE_NUM, BATCH>(a, list.p, cnt, off, s);
+    RT_HIP(hipStreamSynchronize(s));  // slab is released at scope exit
+  } else {
+    HashArgs a = make_args(A, B, bbase);
+    a.crp = C.rp;
+    a.ccol = C.col;
+    a.cval = C.val;
+    launch_hash_bins<MODE_NUM, BATCH>(a, list.p, cnt, off, s);
+    RT_HIP(hipStreamSynchronize(s));
+  }
+}
+
+template <bool BATCH>
+void learned_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key, const i64* cap,
+                  const i64* tmp_off, i32* tcol, float* tval, i64* nnz, i32* ovf,
+                  const Tuning& tu, hipStream_t s) {
+  const i64 m = A.rows;
+  DBuf<i32> list(std::max<i64>(m, 1));
+  int cnt[NBIN], off[NBIN + 1];
+  bin_rows(key, m, LLONG_MAX, tu.num, list.p, cnt, off, s);
+  RT_CHECK(cnt[5] == 0, RT_E_INVALID, "learned rows must fit the LDS hash bins");
+  HashArgs a = make_args(A, B, bbase);
+  a.crp = tmp_off;
+  a.ccol = tcol;
+  a.cval = tval;
+  a.cap = cap;
+  a.ovf = ovf;
+  a.nnz = nnz;
+  launch_hash_bins<MODE_LEARN, BATCH>(a, list.p, cnt, off, s);
+  RT_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int cnt[6],
+              int off[7], hipStream_t s) {
   Thr th;
   for (int b = 0; b < 5; ++b) th.t[b] = th_in[b];
   DBuf<i32> dcnt(NBIN);
@@ -679,92 +847,71 @@ void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int c
   RT_HIP(hipStreamSynchronize(s));
 }
 
-template <bool BATCH>
-void plain_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, const Tuning& tu,
-                hipStream_t s, RunStats* st) {
+void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipStream_t s,
+                  RunStats* st) {
   const i64 m = A.rows;
-  RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
-  C.rows = m;
-  C.cols = B.cols;
-  DBuf<i64> prod(std::max<i64>(m, 1)), nnz(std::max<i64>(m, 1));
   DBuf<unsigned long long> tot(2);
   RT_HIP(hipMemsetAsync(tot.p, 0, 2 * sizeof(unsigned long long), s));
   if (m > 0) {
     ProfScope ps("row_products", s);
-    hipLaunchKernelGGL(k_row_products<BATCH>, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0,
-                       s, m, A.rp, A.col, B.rp, bbase, prod.p, tot.p);
+    const dim3 grid((unsigned)((m * 16 + 255) / 256));
+    if (bbase)
+      hipLaunchKernelGGL(k_row_products<true>, grid, dim3(256), 0, s, m, A.rp, A.col, B.rp, bbase,
+                         prod, tot.p);
+    else
+      hipLaunchKernelGGL(k_row_products<false>, grid, dim3(256), 0, s, m, A.rp, A.col, B.rp,
+                         nullptr, prod, tot.p);
   }
   RT_HIP(hipGetLastError());
-  unsigned long long htot[2];
-  RT_HIP(hipMemcpyAsync(htot, tot.p, sizeof(htot), hipMemcpyDeviceToHost, s));
+  unsigned long long h[2];
+  RT_HIP(hipMemcpyAsync(h, tot.p, sizeof(h), hipMemcpyDeviceToHost, s));
   RT_HIP(hipStreamSynchronize(s));
   if (st) {
-    st->products = (i64)htot[0];
-    st->max_row_products = (i64)htot[1];
-  }
-  RT_HIP(hipMemsetAsync(nnz.p, 0, std::max<i64>(m, 1) * sizeof(i64), s));
-
-  // ---- symbolic ----
-  DBuf<i32> slist(std::max<i64>(m, 1));
-  int scnt[NBIN], soff[NBIN + 1];
-  bin_rows(prod.p, m, B.cols, tu.sym, slist.p, scnt, soff, s);
-  launch_hash_bins<false, BATCH>(slist.p, scnt, soff, A, B, bbase, nnz.p, nullptr, s);
-  const i32 nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
-  DBuf<i32> tilecnt;
-  if (scnt[5] > 0) {
-    tilecnt.alloc((size_t)scnt[5] * nt);
-    const unsigned grid = (unsigned)std::min<i64>(scnt[5], (i64)device().cus * 8);
-    ProfScope ps("sym_bitmap", s);
-    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, slist.p + soff[5],
-                       scnt[5], A.rp, A.col, B.rp, B.col, bbase, B.cols, nt, tilecnt.p, nnz.p);
-    RT_HIP(hipGetLastError());
-  }
-
-  // ---- row pointers ----
-  C.rp = static_cast<i64*>(dmalloc((m + 1) * sizeof(i64)));
-  exclusive_scan_i64(nnz.p, C.rp, m, s);
-  C.nnz = read_scalar(C.rp + m, s);
-  C.col = static_cast<i32*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(i32)));
-  C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
-
-  // ---- numeric ----
-  DBuf<i32> nlist(std::max<i64>(m, 1));
-  int ncnt[NBIN], noff[NBIN + 1];
-  bin_rows(nnz.p, m, LLONG_MAX, tu.num, nlist.p, ncnt, noff, s);
-  if (ncnt[5] > 0) {
-    RT_CHECK(scnt[5] > 0, RT_E_INVALID, "tuning: numeric tile rows without symbolic tile counts");
-    DBuf<i32> slab((size_t)B.rows * (nt + 1));
-    {
-      const i64 nthreads = B.rows * (i64)(nt + 1);
-      ProfScope ps("slab_ptrs", s);
-      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
-                         B.rows, nt, B.rp, B.col, slab.p);
-    }
-    const i64 nitems = (i64)scnt[5] * nt;
-    const unsigned grid = (unsigned)std::min<i64>(nitems, (i64)device().cus * 4);
-    {
-      ProfScope ps("num_tile", s);
-      hipLaunchKernelGGL(k_num_tile<BATCH>, dim3(grid), dim3(1024), 0, s, slist.p + soff[5],
-                         scnt[5], nt, tilecnt.p, nnz.p, tu.num[4], A.rp, A.col, A.val, B.rp,
-                         B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
-    }
-    RT_HIP(hipGetLastError());
-    launch_hash_bins<true, BATCH>(nlist.p, ncnt, noff, A, B, bbase, nullptr, &C, s);
-    RT_HIP(hipStreamSynchronize(s));
-  } else {
-    launch_hash_bins<true, BATCH>(nlist.p, ncnt, noff, A, B, bbase, nullptr, &C, s);
-    RT_HIP(hipStreamSynchronize(s));
+    st->products = (i64)h[0];
+    st->max_row_products = (i64)h[1];
   }
 }
 
-}  // namespace
+void symbolic_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key, i64* nnz,
+                   const Tuning& tu, hipStream_t s, HeavyRows& heavy) {
+  if (bbase) symbolic_impl<true>(A, B, bbase, key, nnz, tu, s, heavy);
+  else symbolic_impl<false>(A, B, nullptr, key, nnz, tu, s, heavy);
+}
+
+void numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                  const HeavyRows& heavy, CsrOut& C, const Tuning& tu, hipStream_t s) {
+  if (bbase) numeric_impl<true>(A, B, bbase, key, heavy, C, tu, s);
+  else numeric_impl<false>(A, B, nullptr, key, heavy, C, tu, s);
+}
+
+void learned_numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                          const i64* cap, const i64* tmp_off, i32* tcol, float* tval, i64* nnz,
+                          i32* ovf, const Tuning& tu, hipStream_t s) {
+  if (bbase) learned_impl<true>(A, B, bbase, key, cap, tmp_off, tcol, tval, nnz, ovf, tu, s);
+  else learned_impl<false>(A, B, nullptr, key, cap, tmp_off, tcol, tval, nnz, ovf, tu, s);
+}
+
+void alloc_c(CsrOut& C, const i64* nnz, i64 m, i64 cols, hipStream_t s) {
+  C.rows = m;
+  C.cols = cols;
+  C.rp = static_cast<i64*>(dmalloc((m + 1) * sizeof(i64)));
+  exclusive_scan_i64(nnz, C.rp, m, s);
+  C.nnz = read_scalar(C.rp + m, s);
+  C.col = static_cast<i32*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(i32)));
+  C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
+}
 
 void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                   const Tuning& tune, hipStream_t s, RunStats* st) {
-  if (batch.b_base)
-    plain_impl<true>(A, B, batch.b_base, C, tune, s, st);
-  else
-    plain_impl<false>(A, B, nullptr, C, tune, s, st);
+  const i64 m = A.rows;
+  RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
+  DBuf<i64> prod(std::max<i64>(m, 1)), nnz(std::max<i64>(m, 1));
+  row_products(A, B, batch.b_base, prod.p, s, st);
+  RT_HIP(hipMemsetAsync(nnz.p, 0, std::max<i64>(m, 1) * sizeof(i64), s));
+  HeavyRows heavy;
+  symbolic_pass(A, B, batch.b_base, prod.p, nnz.p, tune, s, heavy);
+  alloc_c(C, nnz.p, m, B.cols, s);
+  numeric_pass(A, B, batch.b_base, nnz.p, heavy, C, tune, s);
 }
 
 }  // namespace rt

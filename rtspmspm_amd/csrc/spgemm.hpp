@@ -45,6 +45,31 @@ struct RunStats {
   i64 predicted_rows = 0;
 };
 
+// Rows the symbolic pass handled with the LDS-bitmap kernel, with their
+// per-32768-column-tile nnz (used by the dense-tile numeric kernel).
+struct HeavyRows {
+  DBuf<i32> list;  // all binned rows; the heavy ones start at `off`
+  i32 off = 0, n = 0, nt = 1;
+  DBuf<i32> tilecnt;  // n x nt
+};
+
+// Pipeline pieces.  `key` arrays select and size rows: rows with key 0 are
+// skipped.  Symbolic keys are upper bounds (products); numeric keys are the
+// exact nnz; learned keys are predicted capacities.
+void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipStream_t s,
+                  RunStats* st);
+void bin_rows(const i64* x, i64 n, i64 cap, const i64 th[5], i32* list, int cnt[6], int off[7],
+              hipStream_t s);
+void symbolic_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key, i64* nnz,
+                   const Tuning& tu, hipStream_t s, HeavyRows& heavy);
+void numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                  const HeavyRows& heavy, CsrOut& C, const Tuning& tu, hipStream_t s);
+void learned_numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                          const i64* cap, const i64* tmp_off, i32* tcol, float* tval, i64* nnz,
+                          i32* ovf, const Tuning& tu, hipStream_t s);
+// C.rp = exclusive scan of nnz; allocates C.col / C.val.
+void alloc_c(CsrOut& C, const i64* nnz, i64 m, i64 cols, hipStream_t s);
+
 // Exact two-phase product.  C.rp/col/val allocated here.
 void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                   const Tuning& tune, hipStream_t s, RunStats* st);
@@ -67,12 +92,5 @@ void exclusive_scan_i64(const i64* in, i64* out /* n+1 */, i64 n, hipStream_t s)
 // consecutive rows); used for the per-row B base and column limits of batches.
 void fill_segments(i32 count, const i64* seg_rows, const i64* values, i64* out_dev,
                    i64 total_rows, hipStream_t s);
-
-// Predictor state (weights live on the host; uploaded per call).
-struct Mlp;
-const float* mlp_params_host();
-float mlp_margin();
-void mlp_predict_dev(const float* params_dev, const float* feat, i64 n, float* out,
-                     hipStream_t s);
 
 }  // namespace rt
