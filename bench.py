@@ -111,7 +111,17 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the library's device_copy kernel over BYTES (counter "
+                         "calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import rtspmspm_amd as rt
+        for _ in range(3):
+            ms = rt.device_copy_probe(args.copy_calibration)
+        log(f"device_copy {args.copy_calibration} B: {ms:.3f} ms, "
+            f"{2 * args.copy_calibration / ms / 1e6:.1f} GB/s (read + write)")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU row "
               "partition is a design note, not a measured path.", file=sys.stderr)

@@ -127,6 +127,13 @@ int rt_csr_to_device(const rt_csr* host, rt_csr* dev);   /* dev: owner = 1  */
 int rt_csr_to_host(const rt_csr* dev, rt_csr* host);     /* host: owner = 1 */
 /* Plain device -> host copy of `bytes` bytes (e.g. only C.row_ptr).        */
 int rt_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+/* Device -> device copy with the library's streaming copy kernel
+ * ("device_copy"; 16-byte vector loads/stores).  Also the reference point
+ * for reading hardware byte counters.  *ms: kernel time (may be NULL).     */
+int rt_device_copy(void* dst, const void* src, int64_t bytes, double* ms);
+/* Device buffer from the library pool (and back), for callers without HIP. */
+int rt_device_alloc(void** ptr, int64_t bytes);
+int rt_device_free(void* ptr);
 int rt_device_synchronize(void);
 /* Return cached device blocks of the library's pool to the driver.         */
 int rt_pool_trim(void);

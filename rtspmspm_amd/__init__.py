@@ -71,6 +71,9 @@ _sig = {
     "rt_csr_to_device": ([_P(rt_csr), _P(rt_csr)], _c.c_int),
     "rt_csr_to_host": ([_P(rt_csr), _P(rt_csr)], _c.c_int),
     "rt_memcpy_d2h": ([_c.c_void_p, _c.c_void_p, _c.c_int64], _c.c_int),
+    "rt_device_copy": ([_c.c_void_p, _c.c_void_p, _c.c_int64, _P(_c.c_double)], _c.c_int),
+    "rt_device_alloc": ([_P(_c.c_void_p), _c.c_int64], _c.c_int),
+    "rt_device_free": ([_c.c_void_p], _c.c_int),
     "rt_device_synchronize": ([], _c.c_int),
     "rt_pool_trim": ([], _c.c_int),
     "rt_device_info": ([_c.c_char_p, _c.c_int64, _P(_c.c_int64), _P(_c.c_int32)], _c.c_int),
@@ -252,6 +255,21 @@ def spmspm_batched(a_rows, b_rows, b_cols, A_stack: DeviceCSR, B_stack: DeviceCS
                                 b_cols.ctypes.data_as(_P(_c.c_int64)), ctypes.byref(A_stack._s),
                                 ctypes.byref(B_stack._s), ctypes.byref(C), ctypes.byref(o), None))
     return DeviceCSR(C), _info(o)
+
+
+def device_copy_probe(nbytes: int) -> float:
+    """Time the library's streaming copy kernel over nbytes (device to device);
+    returns ms.  Used to calibrate hardware byte counters."""
+    src, dst = _c.c_void_p(), _c.c_void_p()
+    _check(_L.rt_device_alloc(ctypes.byref(src), nbytes))
+    _check(_L.rt_device_alloc(ctypes.byref(dst), nbytes))
+    try:
+        ms = _c.c_double()
+        _check(_L.rt_device_copy(dst, src, nbytes, ctypes.byref(ms)))
+        return ms.value
+    finally:
+        _L.rt_device_free(src)
+        _L.rt_device_free(dst)
 
 
 def synchronize() -> None:

@@ -18,6 +18,7 @@ int prof_count();
 bool prof_get(int i, std::string& name, i64& launches, double& ms);
 void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
                       const char* what);
+double device_copy(void* dst, const void* src, i64 bytes, hipStream_t s);
 }  // namespace rt
 
 using namespace rt;
@@ -364,6 +365,27 @@ int rt_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
       RT_HIP(hipStreamSynchronize(s));
     }
   });
+}
+
+int rt_device_copy(void* dst, const void* src, int64_t bytes, double* ms) {
+  return guard([&] {
+    RT_CHECK(bytes >= 0 && (bytes == 0 || (dst && src)), RT_E_INVALID, "bad copy arguments");
+    device();
+    const double t = bytes ? device_copy(dst, src, bytes, lib_stream()) : 0.0;
+    if (ms) *ms = t;
+  });
+}
+
+int rt_device_alloc(void** ptr, int64_t bytes) {
+  return guard([&] {
+    RT_CHECK(ptr && bytes >= 0, RT_E_INVALID, "bad allocation arguments");
+    device();
+    *ptr = dmalloc((size_t)bytes);
+  });
+}
+
+int rt_device_free(void* ptr) {
+  return guard([&] { dfree(ptr); });
 }
 
 int rt_device_synchronize(void) {

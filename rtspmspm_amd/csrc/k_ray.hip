@@ -219,3 +219,44 @@ void spgemm_ray(const Csr& A, const Csr& B, CsrOut& C, int ablation, int output,
 }
 
 }  // namespace rt
+
+// ---------------------------------------------------------------------------
+// Streaming device copy (16-byte vectors, grid-stride; byte tail handled by
+// the first thread).  Used by rt_device_copy.
+namespace rt {
+__global__ __launch_bounds__(256) void k_device_copy(uint4* __restrict__ dst,
+                                                     const uint4* __restrict__ src, i64 n16,
+                                                     unsigned char* __restrict__ dtail,
+                                                     const unsigned char* __restrict__ stail,
+                                                     int ntail) {
+  const i64 stride = (i64)gridDim.x * 256;
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int t = 0; t < ntail; ++t) dtail[t] = stail[t];
+}
+
+double device_copy(void* dst, const void* src, i64 bytes, hipStream_t s) {
+  RT_CHECK(((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0, RT_E_INVALID,
+           "device_copy needs 16-byte aligned pointers");
+  const i64 n16 = bytes / 16;
+  const int ntail = (int)(bytes - n16 * 16);
+  const unsigned grid = (unsigned)std::max<i64>(1, std::min<i64>((n16 + 255) / 256,
+                                                                  (i64)device().cus * 16));
+  hipEvent_t a, b;
+  RT_HIP(hipEventCreate(&a));
+  RT_HIP(hipEventCreate(&b));
+  RT_HIP(hipEventRecord(a, s));
+  hipLaunchKernelGGL(k_device_copy, dim3(grid), dim3(256), 0, s, static_cast<uint4*>(dst),
+                     static_cast<const uint4*>(src), n16,
+                     static_cast<unsigned char*>(dst) + n16 * 16,
+                     static_cast<const unsigned char*>(src) + n16 * 16, ntail);
+  RT_HIP(hipGetLastError());
+  RT_HIP(hipEventRecord(b, s));
+  RT_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  RT_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+}  // namespace rt
