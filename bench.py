@@ -50,10 +50,11 @@ def kernel_bytes(name, A, P, cnnz, ncols):
         return (key > lo) & (key <= hi)
 
     sizes = {"t32": 0, "t128": 1, "t512": 2, "t2048": 3, "t8192": 4}
-    if name.startswith("num_hash_"):
+    if name.startswith("num_hash_") or name.startswith("learn_hash_"):
+        # learned tables are binned by predicted capacity (~ nnz); nnz bins here
         m = binmask(cnnz, sizes[name.split("_")[-1]])
         return float(16 * m.sum() + 8 * anz[m].sum() + 8 * cnnz[m].sum())
-    if name == "num_tile":
+    if name == "num_heavy":
         m = cnnz > TH[4]
         return float(16 * m.sum() + 8 * anz[m].sum() + 8 * cnnz[m].sum())
     if name.startswith("sym_hash_"):
@@ -107,7 +108,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--method", default="plain", choices=["plain", "learned", "ray"])
+    ap.add_argument("--method", default="learned", choices=["plain", "learned", "ray"])
+    ap.add_argument("--no-compare", dest="compare", action="store_false",
+                    help="skip timing the plain two-phase comparator")
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
@@ -140,29 +143,46 @@ def main():
         f"(gen {time.perf_counter() - t0:.1f} s)")
     dA = rt.to_device(A)
 
-    ms_dev = []
-    C = None
-    for _ in range(args.warmup):
-        C, info = rt.spmspm(dA, dA, mode=args.method)
-        C.free()
-    rt.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        C, info = rt.spmspm(dA, dA, mode=args.method)
-        ms_dev.append(info["ms"])
-        if s + 1 < args.steps:
+    def timed(method):
+        """Warm-up, then K timed steps; returns (wall ms/step, device ms list,
+        last C's per-row nnz, last info)."""
+        for _ in range(args.warmup):
+            C, info = rt.spmspm(dA, dA, mode=method)
             C.free()
-    rt.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
-    cnnz_row = np.diff(C.row_ptr_host())
+        rt.synchronize()
+        ms_dev = []
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            C, info = rt.spmspm(dA, dA, mode=method)
+            ms_dev.append(info["ms"])
+            if s + 1 < args.steps:
+                C.free()
+        rt.synchronize()
+        wall = time.perf_counter() - t0
+        rows = np.diff(C.row_ptr_host())
+        C.free()
+        return wall / args.steps * 1e3, ms_dev, rows, info
+
+    compare = {}
+    if args.compare and args.method != "plain":
+        p_ms, p_dev, p_rows, _ = timed("plain")
+        compare = {"plain_ms_per_step": round(p_ms, 3),
+                   "plain_gflops": round(2.0 * ptot / (p_ms / 1e3) / 1e9, 3)}
+        log(f"plain: {p_ms:.2f} ms/step, {compare['plain_gflops']:.1f} GFLOP/s")
+    ms_per_step, ms_dev, cnnz_row, info = timed(args.method)
+    if "plain_ms_per_step" in compare:
+        assert np.array_equal(p_rows, cnnz_row), "learned and plain row structure differ"
+        compare["speedup_vs_plain"] = round(compare["plain_ms_per_step"] / ms_per_step, 4)
+        compare["overflow_rows"] = info["overflow_rows"]
+        compare["predicted_rows"] = info["predicted_rows"]
     nnz_c = int(cnnz_row.sum())
-    C.free()
     m = A.shape[0]
     bytes_alg = 8 * (m + 1) + 8 * A.nnz + 8 * (m + 1) + 8 * nnz_c
     gflops = 2.0 * ptot / (ms_per_step / 1e3) / 1e9
     log(f"{args.method}: {ms_per_step:.2f} ms/step (device {np.median(ms_dev):.2f} ms median), "
-        f"nnz(C)={nnz_c}, {gflops:.1f} GFLOP/s, pipeline {bytes_alg / ms_per_step / 1e6:.1f} GB/s")
+        f"nnz(C)={nnz_c}, {gflops:.1f} GFLOP/s, pipeline {bytes_alg / ms_per_step / 1e6:.1f} GB/s"
+        + (f", {compare['speedup_vs_plain']:.3f}x plain, overflow rows "
+           f"{info['overflow_rows']}/{info['predicted_rows']}" if compare else ""))
 
     roofline = None
     kernels = {}
@@ -206,7 +226,7 @@ def main():
                    "bytes_alg": bytes_alg,
                    "pipeline_gbs": round(bytes_alg / ms_per_step / 1e6, 2),
                    "pipeline_roofline_frac": round(bytes_alg / ms_per_step / 1e6 / HBM_PEAK_GBS, 5),
-                   "device_ms_median": round(float(np.median(ms_dev)), 3)},
+                   "device_ms_median": round(float(np.median(ms_dev)), 3), **compare},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

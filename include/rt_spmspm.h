@@ -116,6 +116,24 @@ int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_row
                       const rt_csr* B_stack, rt_csr* C_stack,
                       rt_opts* opts, void* stream);
 
+/* ---- numeric-only re-run on a fixed structure ---------------------------
+ * Counterpart of cuSPARSE's SpGEMMreuse path used by the reference's
+ * comparator when built with REUSE (cuSparse/src/main.cpp:98-363, switch at
+ * :648-658): the structure of C (symbolic pass, row binning, long-row tile
+ * counts) is computed once by rt_plan_create, which also returns C; each
+ * rt_plan_execute recomputes C's values from new values of A and B with the
+ * same structure (same row_ptr / col of A and B).  Device operands only;
+ * A's and B's row_ptr/col arrays are borrowed by the plan until it is
+ * destroyed.                                                                 */
+typedef struct rt_plan rt_plan;
+int rt_plan_create(const rt_csr* A, const rt_csr* B, rt_plan** plan, rt_csr* C,
+                   void* stream);
+/* a_val / b_val: device arrays of A.nnz / B.nnz new values; C must be the
+ * rt_csr returned by rt_plan_create (its val array is overwritten).         */
+int rt_plan_execute(rt_plan* plan, const float* a_val, const float* b_val, rt_csr* C,
+                    double* ms);
+void rt_plan_destroy(rt_plan* plan);
+
 /* Release arrays that the library allocated (owner == 1); zeroes *C.       */
 void rt_csr_free(rt_csr* C);
 

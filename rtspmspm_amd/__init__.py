@@ -66,6 +66,9 @@ _sig = {
     "rt_spmspm_ex": ([_P(rt_csr), _P(rt_csr), _P(rt_csr), _P(rt_opts), _c.c_void_p], _c.c_int),
     "rt_spmspm_batched": ([_c.c_int32, _P(_c.c_int64), _P(_c.c_int64), _P(_c.c_int64),
                            _P(rt_csr), _P(rt_csr), _P(rt_csr), _P(rt_opts), _c.c_void_p], _c.c_int),
+    "rt_plan_create": ([_P(rt_csr), _P(rt_csr), _P(_c.c_void_p), _P(rt_csr), _c.c_void_p], _c.c_int),
+    "rt_plan_execute": ([_c.c_void_p, _c.c_void_p, _c.c_void_p, _P(rt_csr), _P(_c.c_double)], _c.c_int),
+    "rt_plan_destroy": ([_c.c_void_p], None),
     "rt_csr_free": ([_P(rt_csr)], None),
     "rt_last_error": ([], _c.c_char_p),
     "rt_csr_to_device": ([_P(rt_csr), _P(rt_csr)], _c.c_int),
@@ -255,6 +258,38 @@ def spmspm_batched(a_rows, b_rows, b_cols, A_stack: DeviceCSR, B_stack: DeviceCS
                                 b_cols.ctypes.data_as(_P(_c.c_int64)), ctypes.byref(A_stack._s),
                                 ctypes.byref(B_stack._s), ctypes.byref(C), ctypes.byref(o), None))
     return DeviceCSR(C), _info(o)
+
+
+class Plan:
+    """Fixed-structure product (rt_plan_*): the symbolic pass runs once at
+    construction; execute() recomputes C's values for new A/B values."""
+
+    def __init__(self, A: DeviceCSR, B: DeviceCSR):
+        self._p = _c.c_void_p()
+        C = rt_csr()
+        _check(_L.rt_plan_create(ctypes.byref(A._s), ctypes.byref(B._s), ctypes.byref(self._p),
+                                 ctypes.byref(C), None))
+        self.C = DeviceCSR(C)
+        self._operands = (A, B)  # the plan borrows their structure arrays
+
+    def execute(self, A_values: DeviceCSR, B_values: DeviceCSR) -> float:
+        """New values are taken from the val arrays of two device CSRs with
+        the plan's structure; returns the device time in ms."""
+        ms = _c.c_double()
+        _check(_L.rt_plan_execute(self._p, A_values._s.val, B_values._s.val,
+                                  ctypes.byref(self.C._s), ctypes.byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if self._p:
+            _L.rt_plan_destroy(self._p)
+            self._p = _c.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_copy_probe(nbytes: int) -> float:

@@ -100,6 +100,22 @@ __global__ __launch_bounds__(256) void k_bin_fill(i64 n, const i64* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Product p of the current chunk -> entry index, from the previous entry
+// `lo`: one LDS read when the entry still covers p (long B rows), a binary
+// search over the rest otherwise (short B rows).
+__device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo, int nvalid,
+                                             int p) {
+  if (lo + 1 < nvalid && ex[lo + 1] <= p) {
+    int a = lo + 1, b = nvalid - 1;
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (ex[mid] <= p) a = mid; else b = mid - 1;
+    }
+    lo = a;
+  }
+  return lo;
+}
+
 // LDS-hash row kernel.  A group of G lanes owns one output row and a table of
 // T slots (linear probing, load factor <= 1/2 by binning).  The products of
 // the row are enumerated flat: a chunk of up to G entries A(i,k) is loaded,
@@ -212,11 +228,7 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
     const int nvalid = (int)min((i64)G, a1 - e0);
     int lo = 0;
     for (int p = l; p < total; p += G) {
-      int hi = nvalid - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_ex[gb + mid] <= p) lo = mid; else hi = mid - 1;
-      }
+      lo = advance_entry(s_ex + gb, lo, nvalid, p);
       const i64 idx = s_bs[gb + lo] + (p - s_ex[gb + lo]);
       const i32 c = bcol[idx];
       u32 h = hash_col<LT>(c);
@@ -390,11 +402,7 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
         const int nvalid = (int)min((i64)WG, a1 - e0);
         int lo = 0;
         for (int p = l; p < tot; p += WG) {
-          int hi = nvalid - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_ex[mid] <= p) lo = mid; else hi = mid - 1;
-          }
+          lo = advance_entry(s_ex, lo, nvalid, p);
           const i64 c = (i64)bcol[s_bs[lo] + (p - s_ex[lo])] - win;
           if (c >= 0 && c < wcols) lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
         }
@@ -444,16 +452,74 @@ __global__ __launch_bounds__(256) void k_slab_ptrs(i64 brows, i32 nt, const i64*
   slab[gid] = (i32)res;
 }
 
-// Dense LDS column-tile numeric kernel for long rows.  Work item = (row,
-// tile); the tile's 32768 fp32 accumulators and presence bits live in LDS,
+// Dense LDS column-tile numeric kernel for long rows.  One workgroup per row
+// (grid-stride over the rows the LDS-bitmap symbolic kernel handled, keeping
+// those whose key exceeds `th`).  The row's 32768-column tiles are processed
+// in order: a tile's 32768 fp32 accumulators and presence bits live in LDS,
 // the row's products restricted to the tile are read through the B slab
 // pointers, and the tile is emitted in column order by a scan over the
-// presence bits (no sort needed).  LDS is re-zeroed entry by entry as it is
-// emitted so that the next item starts clean.
+// presence bits (no sort), at a running output offset.  The A row (up to
+// HCH*1024 entries) is read once and kept in registers across tiles, the
+// next tile's slab boundaries are prefetched while the current tile runs,
+// and tiles the symbolic pass found empty are skipped without a barrier.
+constexpr int HCH = 4;
+constexpr int TC_LDS = 256;  // tile counts staged in LDS (ncols <= 8M)
+
+// Accumulates the products of one chunk of A entries (len/start/a given per
+// thread) into the LDS tile.  Two block-wide barriers.
+template <int WG>
+__device__ __forceinline__ void tile_chunk(int len, i64 start, float av, i64 cbase,
+                                           const i32* __restrict__ bcol,
+                                           const float* __restrict__ bval, float* s_vals,
+                                           u32* s_bits, i64* s_bs, i32* s_ex, float* s_av,
+                                           i32* s_scr, int nvalid) {
+  const int l = threadIdx.x;
+  int tot;
+  const int inc = block_scan_incl<WG>(len, s_scr, &tot);
+  s_ex[l] = inc - len;
+  s_bs[l] = start;
+  s_av[l] = av;
+  __syncthreads();
+  int lo = 0;
+  for (int p = l; p < tot; p += WG) {
+    lo = advance_entry(s_ex, lo, nvalid, p);
+    const i64 idx = s_bs[lo] + (p - s_ex[lo]);
+    const int c = (int)(bcol[idx] - cbase);
+    lds_add_f32(&s_vals[c], s_av[lo] * bval[idx]);
+    lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+  }
+  __syncthreads();
+}
+
+// Emits the finished LDS tile at `out` in column order and re-zeroes it.
+template <int WG>
+__device__ __forceinline__ void tile_emit(i64 out, i64 cbase, float* s_vals, u32* s_bits,
+                                          i32* s_scr, i32* __restrict__ ccol,
+                                          float* __restrict__ cval) {
+  const int l = threadIdx.x;
+  const u32 bits = s_bits[l];
+  const int pc = __popc(bits);
+  int tot;
+  const int inc = block_scan_incl<WG>(pc, s_scr, &tot);
+  i64 pos = out + (inc - pc);
+  u32 b = bits;
+  while (b) {
+    const int bi = __ffs(b) - 1;
+    b &= b - 1;
+    const int cl = l * 32 + bi;
+    ccol[pos] = (i32)(cbase + cl);
+    cval[pos] = s_vals[cl];
+    s_vals[cl] = 0.f;
+    ++pos;
+  }
+  s_bits[l] = 0;
+  __syncthreads();
+}
+
 template <bool BATCH>
-__global__ __launch_bounds__(1024) void k_num_tile(
+__global__ __launch_bounds__(1024) void k_num_heavy(
     const i32* __restrict__ list, i32 nlist, i32 nt, const i32* __restrict__ tilecnt,
-    const i64* __restrict__ nnz_row, i64 th, const i64* __restrict__ arp,
+    const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const float* __restrict__ aval,
     const i64* __restrict__ brp, const i32* __restrict__ bcol,
     const float* __restrict__ bval, const i64* __restrict__ bbase,
@@ -467,78 +533,94 @@ __global__ __launch_bounds__(1024) void k_num_tile(
   __shared__ i32 s_ex[WG];
   __shared__ float s_av[WG];
   __shared__ i32 s_scr[WG / WAVE];
-  __shared__ i64 s_off;
+  __shared__ i32 s_tc[TC_LDS];
   const int l = threadIdx.x;
   for (int d = l; d < TILE_W; d += WG) s_vals[d] = 0.f;
   s_bits[l] = 0;
   __syncthreads();
-  const i64 nitems = (i64)nlist * nt;
-  for (i64 item = blockIdx.x; item < nitems; item += gridDim.x) {
-    const i64 h = item / nt;
-    const int t = (int)(item - h * nt);
+  for (i64 h = blockIdx.x; h < nlist; h += gridDim.x) {
     const i32 row = list[h];
-    if (nnz_row[row] <= th) continue;
-    if (tilecnt[h * nt + t] == 0) continue;
-    if (l < WAVE) {
-      i64 s = 0;
-      for (int q = l; q < t; q += WAVE) s += tilecnt[h * nt + q];
-      s = seg_reduce_sum<WAVE>(s);
-      if (l == 0) s_off = crp[row] + s;
-    }
-    const i64 a0 = arp[row], a1 = arp[row + 1];
+    if (key[row] <= th) continue;
+    const i32* tc = tilecnt + h * nt;
+    for (int t = l; t < min(nt, TC_LDS); t += WG) s_tc[t] = tc[t];
+    const i64 a0 = arp[row], a1 = arp[row + 1], nA = a1 - a0;
     const i64 bb = BATCH ? bbase[row] : 0;
-    const i64 cbase = (i64)t * TILE_W;
-    for (i64 e0 = a0; e0 < a1; e0 += WG) {
-      const i64 e = e0 + l;
-      int len = 0;
-      i64 bs = 0;
-      float av = 0.f;
-      if (e < a1) {
-        const i64 k = (i64)acol[e] + bb;
-        const i32* sp = slab + k * (nt + 1);
-        const i32 s0 = sp[t], s1 = sp[t + 1];
-        bs = brp[k] + s0;
-        len = s1 - s0;
-        av = aval[e];
-      }
-      int tot;
-      const int inc = block_scan_incl<WG>(len, s_scr, &tot);
-      s_ex[l] = inc - len;
-      s_bs[l] = bs;
-      s_av[l] = av;
-      __syncthreads();
-      const int nvalid = (int)min((i64)WG, a1 - e0);
-      int lo = 0;
-      for (int p = l; p < tot; p += WG) {
-        int hi = nvalid - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_ex[mid] <= p) lo = mid; else hi = mid - 1;
-        }
-        const i64 idx = s_bs[lo] + (p - s_ex[lo]);
-        const int c = (int)(bcol[idx] - cbase);
-        lds_add_f32(&s_vals[c], s_av[lo] * bval[idx]);
-        lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
-      }
-      __syncthreads();
-    }
-    const u32 bits = s_bits[l];
-    const int pc = __popc(bits);
-    int tot;
-    const int inc = block_scan_incl<WG>(pc, s_scr, &tot);
-    i64 pos = s_off + (inc - pc);
-    u32 b = bits;
-    while (b) {
-      const int bi = __ffs(b) - 1;
-      b &= b - 1;
-      const int cl = l * 32 + bi;
-      ccol[pos] = (i32)(cbase + cl);
-      cval[pos] = s_vals[cl];
-      s_vals[cl] = 0.f;
-      ++pos;
-    }
-    s_bits[l] = 0;
+    i64 out = crp[row];
     __syncthreads();
+    if (nA <= (i64)HCH * WG) {
+      const int nch = (int)((nA + WG - 1) / WG);
+      bool ok[HCH];
+      float av[HCH];
+      i64 b0[HCH];
+      const i32* sp[HCH];
+      i32 cur[HCH], nxt[HCH];
+#pragma unroll
+      for (int c = 0; c < HCH; ++c) {
+        const i64 e = a0 + (i64)c * WG + l;
+        ok[c] = e < a1;
+        av[c] = 0.f;
+        b0[c] = 0;
+        sp[c] = slab;
+        cur[c] = nxt[c] = 0;
+        if (ok[c]) {
+          const i64 k = (i64)acol[e] + bb;
+          av[c] = aval[e];
+          b0[c] = brp[k];
+          sp[c] = slab + k * (nt + 1);
+          nxt[c] = sp[c][1];
+        }
+      }
+      for (int t = 0; t < nt; ++t) {
+        const int cnt = t < TC_LDS ? s_tc[t] : tc[t];
+        i32 pre[HCH];
+#pragma unroll
+        for (int c = 0; c < HCH; ++c) pre[c] = (ok[c] && t + 2 <= nt) ? sp[c][t + 2] : 0;
+        if (cnt > 0) {
+          const i64 cbase = (i64)t * TILE_W;
+#pragma unroll
+          for (int c = 0; c < HCH; ++c) {
+            if (c < nch) {
+              const int nvalid = (int)min((i64)WG, nA - (i64)c * WG);
+              tile_chunk<WG>(ok[c] ? nxt[c] - cur[c] : 0, b0[c] + cur[c], av[c], cbase, bcol,
+                             bval, s_vals, s_bits, s_bs, s_ex, s_av, s_scr, nvalid);
+            }
+          }
+          tile_emit<WG>(out, cbase, s_vals, s_bits, s_scr, ccol, cval);
+          out += cnt;
+        }
+#pragma unroll
+        for (int c = 0; c < HCH; ++c) {
+          cur[c] = nxt[c];
+          nxt[c] = pre[c];
+        }
+      }
+    } else {
+      // Very long A rows: slab boundaries are read per tile and chunk.
+      for (int t = 0; t < nt; ++t) {
+        const int cnt = t < TC_LDS ? s_tc[t] : tc[t];
+        if (cnt == 0) continue;
+        const i64 cbase = (i64)t * TILE_W;
+        for (i64 e0 = a0; e0 < a1; e0 += WG) {
+          const i64 e = e0 + l;
+          int len = 0;
+          i64 start = 0;
+          float a = 0.f;
+          if (e < a1) {
+            const i64 k = (i64)acol[e] + bb;
+            const i32* sk = slab + k * (nt + 1);
+            const i32 s0 = sk[t], s1 = sk[t + 1];
+            start = brp[k] + s0;
+            len = s1 - s0;
+            a = aval[e];
+          }
+          tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_bits, s_bs, s_ex, s_av,
+                         s_scr, (int)min((i64)WG, a1 - e0));
+        }
+        tile_emit<WG>(out, cbase, s_vals, s_bits, s_scr, ccol, cval);
+        out += cnt;
+      }
+    }
+    __syncthreads();  // s_tc is reloaded for the next row
   }
 }
 
@@ -775,11 +857,10 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
                          B.rows, heavy.nt, B.rp, B.col, slab.p);
       RT_HIP(hipGetLastError());
     }
-    const i64 nitems = (i64)heavy.n * heavy.nt;
-    const unsigned grid = (unsigned)std::min<i64>(nitems, (i64)device().cus * 4);
+    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 4);
     {
-      ProfScope ps("num_tile", s);
-      hipLaunchKernelGGL(k_num_tile<BATCH>, dim3(grid), dim3(1024), 0, s, heavy.list.p + heavy.off,
+      ProfScope ps("num_heavy", s);
+      hipLaunchKernelGGL(k_num_heavy<BATCH>, dim3(grid), dim3(1024), 0, s, heavy.list.p + heavy.off,
                          heavy.n, heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val,
                          B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
       RT_HIP(hipGetLastError());

@@ -1,0 +1,87 @@
+"""GPU tests: numeric-only re-run on a fixed structure (rt_plan_*) and the
+optixSpMSpM command-line drop-in."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import rtspmspm_amd as rt
+from tests.helpers import check_against_golden, check_parity, golden, load_fixture
+
+pytestmark = pytest.mark.gpu
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(REPO, "rtspmspm_amd", "optixSpMSpM")
+
+
+def with_values(M, vals):
+    return rt.CSR(M.shape, M.row_ptr, M.col, np.asarray(vals, np.float32))
+
+
+@pytest.mark.parametrize("make", [lambda: rt.gen_rmat(13, 16, seed=31),
+                                  lambda: rt.gen_uniform_k(30000, 90, 32)])
+def test_plan_reruns_numeric_with_new_values(make):
+    A = make()
+    dA = rt.to_device(A)
+    plan = rt.Plan(dA, dA)
+    check_parity(plan.C.to_host(), oracle.spgemm_f64(A, A))
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        A2 = with_values(A, rng.uniform(-2, 2, A.nnz))
+        B2 = with_values(A, rng.uniform(-2, 2, A.nnz))
+        d2, e2 = rt.to_device(A2), rt.to_device(B2)
+        ms = plan.execute(d2, e2)
+        assert ms > 0
+        check_parity(plan.C.to_host(), oracle.spgemm_f64(A2, B2))
+    plan.close()
+
+
+def test_plan_rejects_foreign_result():
+    A = rt.gen_uniform_k(1000, 4, 33)
+    dA = rt.to_device(A)
+    plan = rt.Plan(dA, dA)
+    other, _ = rt.spmspm(dA, dA)
+    with pytest.raises(rt.RtError):
+        rt._check(rt._L.rt_plan_execute(plan._p, dA._s.val, dA._s.val,
+                                        rt.ctypes.byref(other._s), None))
+    plan.close()
+
+
+@pytest.mark.parametrize("mode,banner", [("ray", "ATOMIC"), ("plain", "PLAIN")])
+def test_cli_matches_reference_contract(tmp_path, mode, banner):
+    name = "wiki-Vote"
+    A = load_fixture(name)
+    a = tmp_path / "a.mtx"
+    rt.mtx_write(str(a), A)
+    out = tmp_path / "c.mtx"
+    log = tmp_path / "log.txt"
+    r = subprocess.run([CLI, "-m1", str(a), "-m2", str(a), "-o", str(out), "-l", str(log),
+                        "--mode", mode], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith(banner) and r.stdout.rstrip().endswith("success!")
+    ms = float(log.read_text())
+    assert ms > 0 and "\n" not in log.read_text()
+    C = rt.mtx_read(str(out))
+    check_against_golden(C, golden()[name])
+
+
+def test_cli_ref_compat_and_errors(tmp_path):
+    A = load_fixture("mario002")
+    a = tmp_path / "a.mtx"
+    rt.mtx_write(str(a), A)
+    out = tmp_path / "c.mtx"
+    r = subprocess.run([CLI, "-m1", str(a), "-m2", str(a), "-o", str(out), "--ref-compat"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    C = rt.mtx_read(str(out))
+    ref = oracle.spgemm_f64(A, A)
+    assert C.nnz == int(np.count_nonzero(ref.val.astype(np.float32)))  # exact zeros dropped
+    r = subprocess.run([CLI, "-m1", str(tmp_path / "missing.mtx"), "-m2", str(a)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Caught exception" in r.stderr
+    rect = tmp_path / "r.mtx"
+    rt.mtx_write(str(rect), rt.gen_fixed_normal(10, 20, 3, 1))
+    r = subprocess.run([CLI, "-m1", str(rect), "-m2", str(rect)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "shape mismatch" in r.stderr
