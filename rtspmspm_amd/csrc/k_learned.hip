@@ -9,11 +9,13 @@
 //      row's product columns, ...),
 //   2. a small MLP (8 -> 32 -> 32 -> 1, ReLU) predicts log2(1 + nnz) of the
 //      row; capacity = 2^(prediction + margin) - 1, clamped to [1, U],
-//   3. the numeric LDS-hash kernels run with tables sized from the capacity
-//      and write each row into a temporary buffer; a row that finds more
-//      distinct columns than its capacity is abandoned and flagged,
-//   4. flagged rows, and rows too long for the LDS-hash tables, go through
-//      the exact symbolic + numeric path; the others are copied into C.
+//   3. short rows run through the numeric LDS-hash kernels with tables sized
+//      from the capacity, long rows through the dense-tile kernel; both write
+//      the row into a temporary buffer at capacity-sized offsets, and a row
+//      that finds more distinct columns than its capacity is abandoned and
+//      flagged,
+//   4. flagged rows go through the exact symbolic + numeric path; the others
+//      are copied into C.
 // C is identical to the exact method's: the structure of every row is
 // computed exactly either way, only the sizing of the tables differs.
 //
@@ -241,29 +243,30 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ p
 }
 
 // ---- capacities, key selection, copy ---------------------------------------
-// cap = clamp(ceil(2^(pred + margin) - 1), 1, U) for rows that fit the LDS
-// hash bins (U <= th); rows beyond go to the exact path (key_exact = P).
+// cap = clamp(ceil(2^(pred + margin) - 1), 1, U) for every row with products;
+// rows with U <= th go to the LDS-hash learned kernels (key_light), longer
+// rows to the learned dense-tile kernel (key_heavy).
 __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__ prod, i64 ncols,
                                                   const float* __restrict__ pred, float margin,
                                                   i64 th, i64* __restrict__ cap,
+                                                  i64* __restrict__ key_light,
+                                                  i64* __restrict__ key_heavy,
                                                   i64* __restrict__ key_exact,
                                                   i32* __restrict__ ovf) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   if (i >= m) return;
   const i64 P = prod[i];
   const i64 U = min(P, ncols);
-  i64 c = 0, ke = 0;
+  i64 c = 0;
   if (P > 0) {
-    if (U > th) {
-      ke = P;
-    } else {
-      const float e = fminf(pred[i] + margin, 40.f);
-      c = (i64)ceilf(exp2f(e) - 1.f);
-      c = max((i64)1, min(c, U));
-    }
+    const float e = fminf(pred[i] + margin, 40.f);
+    c = (i64)ceilf(exp2f(e) - 1.f);
+    c = max((i64)1, min(c, U));
   }
   cap[i] = c;
-  key_exact[i] = ke;
+  key_light[i] = U <= th ? c : 0;
+  key_heavy[i] = U > th ? c : 0;
+  key_exact[i] = 0;
   ovf[i] = 0;
 }
 
@@ -427,7 +430,8 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   const i64 M = std::max<i64>(m, 1);
   DBuf<float> params;
   upload(params, hp.data(), hp.size(), s);
-  DBuf<i64> prod(M), cap(M), key_exact(M), nnz(M), tmp_off(m + 1), key_num(M);
+  DBuf<i64> prod(M), cap(M), key_light(M), key_heavy(M), key_exact(M), nnz(M), tmp_off(m + 1),
+      key_num(M);
   DBuf<i32> ovf(M);
   DBuf<float> feat(M * NF), pred(M);
   RunStats local;
@@ -440,14 +444,17 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   {
     ProfScope ps("capacity", s);
     hipLaunchKernelGGL(k_capacity, g256, dim3(256), 0, s, m, prod.p, B.cols, pred.p, margin,
-                       std::min(tu.sym[4], tu.num[4]), cap.p, key_exact.p, ovf.p);
+                       std::min(tu.sym[4], tu.num[4]), cap.p, key_light.p, key_heavy.p,
+                       key_exact.p, ovf.p);
     RT_HIP(hipGetLastError());
   }
   exclusive_scan_i64(cap.p, tmp_off.p, m, s);
   const i64 tmp_total = read_scalar(tmp_off.p + m, s);
   DBuf<i32> tcol(std::max<i64>(tmp_total, 1));
   DBuf<float> tval(std::max<i64>(tmp_total, 1));
-  learned_numeric_pass(A, B, bbase, cap.p, cap.p, tmp_off.p, tcol.p, tval.p, nnz.p, ovf.p, tu, s);
+  learned_numeric_pass(A, B, bbase, key_light.p, cap.p, tmp_off.p, tcol.p, tval.p, nnz.p, ovf.p,
+                       tu, s);
+  learned_heavy_pass(A, B, bbase, key_heavy.p, cap.p, tmp_off.p, tcol.p, tval.p, nnz.p, ovf.p, s);
   DBuf<unsigned long long> counts(2);
   RT_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s));
   {

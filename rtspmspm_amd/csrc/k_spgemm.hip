@@ -22,6 +22,7 @@ constexpr int NBIN = 6;
 constexpr int TILE_W = 32768;                   // columns per dense numeric tile
 constexpr int WIN_TILES = 32;                   // tiles per symbolic bitmap window
 constexpr i64 WIN_W = (i64)TILE_W * WIN_TILES;  // 2^20 columns per window
+constexpr int UNR = 4;                          // products gathered per lane per step
 
 struct Thr {
   i64 t[5];
@@ -401,10 +402,27 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
         __syncthreads();
         const int nvalid = (int)min((i64)WG, a1 - e0);
         int lo = 0;
-        for (int p = l; p < tot; p += WG) {
-          lo = advance_entry(s_ex, lo, nvalid, p);
-          const i64 c = (i64)bcol[s_bs[lo] + (p - s_ex[lo])] - win;
-          if (c >= 0 && c < wcols) lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+        for (int p0 = l; p0 < tot; p0 += UNR * WG) {
+          i64 idx[UNR];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u) {
+            const int p = p0 + u * WG;
+            idx[u] = -1;
+            if (p < tot) {
+              lo = advance_entry(s_ex, lo, nvalid, p);
+              idx[u] = s_bs[lo] + (p - s_ex[lo]);
+            }
+          }
+          i32 cc[UNR];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u)
+            if (idx[u] >= 0) cc[u] = bcol[idx[u]];
+#pragma unroll
+          for (int u = 0; u < UNR; ++u)
+            if (idx[u] >= 0) {
+              const i64 c = (i64)cc[u] - win;
+              if (c >= 0 && c < wcols) lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+            }
         }
         __syncthreads();
       }
@@ -466,57 +484,129 @@ constexpr int HCH = 4;
 constexpr int TC_LDS = 256;  // tile counts staged in LDS (ncols <= 8M)
 
 // Accumulates the products of one chunk of A entries (len/start/a given per
-// thread) into the LDS tile.  Two block-wide barriers.
+// thread) into the LDS tile; returns the chunk's product count (uniform).
+// Each lane gathers UNR products before using them, so several B loads are
+// in flight per lane.  Two block-wide barriers.
 template <int WG>
-__device__ __forceinline__ void tile_chunk(int len, i64 start, float av, i64 cbase,
-                                           const i32* __restrict__ bcol,
-                                           const float* __restrict__ bval, float* s_vals,
-                                           u32* s_bits, i64* s_bs, i32* s_ex, float* s_av,
-                                           i32* s_scr, int nvalid) {
+__device__ __forceinline__ int tile_chunk(int len, i64 start, float av, i64 cbase,
+                                          const i32* __restrict__ bcol,
+                                          const float* __restrict__ bval, float* s_vals,
+                                          u32* s_bits, i64* s_bs, i32* s_ex, float* s_av,
+                                          i32* s_scr, int nvalid) {
   const int l = threadIdx.x;
   int tot;
   const int inc = block_scan_incl<WG>(len, s_scr, &tot);
+  if (tot == 0) {  // uniform: nothing of this chunk falls in the tile
+    __syncthreads();  // every wave has read s_scr before the next scan writes it
+    return 0;
+  }
   s_ex[l] = inc - len;
   s_bs[l] = start;
   s_av[l] = av;
   __syncthreads();
   int lo = 0;
-  for (int p = l; p < tot; p += WG) {
-    lo = advance_entry(s_ex, lo, nvalid, p);
-    const i64 idx = s_bs[lo] + (p - s_ex[lo]);
-    const int c = (int)(bcol[idx] - cbase);
-    lds_add_f32(&s_vals[c], s_av[lo] * bval[idx]);
-    lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+  for (int p0 = l; p0 < tot; p0 += UNR * WG) {
+    i64 idx[UNR];
+    float a[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int p = p0 + u * WG;
+      idx[u] = -1;
+      a[u] = 0.f;
+      if (p < tot) {
+        lo = advance_entry(s_ex, lo, nvalid, p);
+        idx[u] = s_bs[lo] + (p - s_ex[lo]);
+        a[u] = s_av[lo];
+      }
+    }
+    i32 c[UNR];
+    float v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (idx[u] >= 0) {
+        c[u] = bcol[idx[u]];
+        v[u] = bval[idx[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (idx[u] >= 0) {
+        const int cl = (int)(c[u] - cbase);
+        lds_add_f32(&s_vals[cl], a[u] * v[u]);
+        lds_or_u32(&s_bits[cl >> 5], 1u << (cl & 31));
+      }
   }
   __syncthreads();
+  return tot;
+}
+
+// Position of the r-th (0-based) set bit of b; requires r < popcount(b).
+__device__ __forceinline__ int select_bit(u32 b, int r) {
+  int pos = 0, c;
+  c = __popc(b & 0xFFFFu);
+  if (r >= c) { r -= c; b >>= 16; pos += 16; }
+  c = __popc(b & 0xFFu);
+  if (r >= c) { r -= c; b >>= 8; pos += 8; }
+  c = __popc(b & 0xFu);
+  if (r >= c) { r -= c; b >>= 4; pos += 4; }
+  c = __popc(b & 0x3u);
+  if (r >= c) { r -= c; b >>= 2; pos += 2; }
+  c = (int)(b & 1u);
+  if (r >= c) pos += 1;
+  return pos;
 }
 
 // Emits the finished LDS tile at `out` in column order and re-zeroes it.
+// Output position q is written by lane q mod WG (coalesced): its word comes
+// from a binary search over the per-word popcount prefix, its bit from a
+// select inside the word.  Returns the tile's entry count, or -1 (and
+// writes nothing) when that count exceeds `limit`.
 template <int WG>
-__device__ __forceinline__ void tile_emit(i64 out, i64 cbase, float* s_vals, u32* s_bits,
-                                          i32* s_scr, i32* __restrict__ ccol,
-                                          float* __restrict__ cval) {
+__device__ __forceinline__ int tile_emit(i64 out, i64 cbase, i64 limit, float* s_vals,
+                                         u32* s_bits, i32* s_pfx, i32* s_scr,
+                                         i32* __restrict__ ccol, float* __restrict__ cval) {
+  constexpr int NW = TILE_W / 32;
+  static_assert(NW == WG, "one presence word per thread");
   const int l = threadIdx.x;
   const u32 bits = s_bits[l];
   const int pc = __popc(bits);
   int tot;
   const int inc = block_scan_incl<WG>(pc, s_scr, &tot);
-  i64 pos = out + (inc - pc);
+  s_pfx[l] = inc - pc;
+  __syncthreads();
+  const bool fits = (i64)tot <= limit;
+  if (fits) {
+    int w = 0;
+    for (int q = l; q < tot; q += WG) {
+      int hi = NW - 1;
+      while (w < hi) {
+        const int mid = (w + hi + 1) >> 1;
+        if (s_pfx[mid] <= q) w = mid; else hi = mid - 1;
+      }
+      const int cl = w * 32 + select_bit(s_bits[w], q - s_pfx[w]);
+      ccol[out + q] = (i32)(cbase + cl);
+      cval[out + q] = s_vals[cl];
+    }
+  }
+  __syncthreads();
   u32 b = bits;
   while (b) {
     const int bi = __ffs(b) - 1;
     b &= b - 1;
-    const int cl = l * 32 + bi;
-    ccol[pos] = (i32)(cbase + cl);
-    cval[pos] = s_vals[cl];
-    s_vals[cl] = 0.f;
-    ++pos;
+    s_vals[l * 32 + bi] = 0.f;
   }
   s_bits[l] = 0;
   __syncthreads();
+  return fits ? tot : -1;
 }
 
-template <bool BATCH>
+// LEARN = false: exact rows (key > th) of the symbolic bitmap list, output at
+//   crp[row] of C, empty tiles skipped by the symbolic tile counts.
+// LEARN = true: rows sized by the predictor, output at crp[row] (= the
+//   temporary offsets) with capacity cap[row]; nnz[row] receives the count,
+//   or ovf[row] = 1 when the row outgrows its capacity (the rest of the row
+//   is then abandoned).  No tile counts: a tile whose chunk scans total zero
+//   products is skipped.
+template <bool BATCH, bool LEARN>
 __global__ __launch_bounds__(1024) void k_num_heavy(
     const i32* __restrict__ list, i32 nlist, i32 nt, const i32* __restrict__ tilecnt,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
@@ -524,11 +614,12 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
     const i64* __restrict__ brp, const i32* __restrict__ bcol,
     const float* __restrict__ bval, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
-    float* __restrict__ cval) {
+    float* __restrict__ cval, const i64* __restrict__ cap, i64* __restrict__ nnz,
+    i32* __restrict__ ovf) {
   constexpr int WG = 1024;
-  static_assert(TILE_W / 32 == WG, "one presence word per thread");
   __shared__ float s_vals[TILE_W];
   __shared__ u32 s_bits[TILE_W / 32];
+  __shared__ i32 s_pfx[TILE_W / 32];
   __shared__ i64 s_bs[WG];
   __shared__ i32 s_ex[WG];
   __shared__ float s_av[WG];
@@ -541,11 +632,16 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
   for (i64 h = blockIdx.x; h < nlist; h += gridDim.x) {
     const i32 row = list[h];
     if (key[row] <= th) continue;
-    const i32* tc = tilecnt + h * nt;
-    for (int t = l; t < min(nt, TC_LDS); t += WG) s_tc[t] = tc[t];
+    const i32* tc = LEARN ? nullptr : tilecnt + h * nt;
+    if constexpr (!LEARN) {
+      for (int t = l; t < min(nt, TC_LDS); t += WG) s_tc[t] = tc[t];
+    }
     const i64 a0 = arp[row], a1 = arp[row + 1], nA = a1 - a0;
     const i64 bb = BATCH ? bbase[row] : 0;
-    i64 out = crp[row];
+    const i64 base = crp[row];
+    const i64 capacity = LEARN ? cap[row] : LLONG_MAX;
+    i64 out = base;
+    bool over = false;
     __syncthreads();
     if (nA <= (i64)HCH * WG) {
       const int nch = (int)((nA + WG - 1) / WG);
@@ -570,23 +666,27 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
           nxt[c] = sp[c][1];
         }
       }
-      for (int t = 0; t < nt; ++t) {
-        const int cnt = t < TC_LDS ? s_tc[t] : tc[t];
+      for (int t = 0; t < nt && !over; ++t) {
+        const int cnt = LEARN ? 1 : (t < TC_LDS ? s_tc[t] : tc[t]);
         i32 pre[HCH];
 #pragma unroll
         for (int c = 0; c < HCH; ++c) pre[c] = (ok[c] && t + 2 <= nt) ? sp[c][t + 2] : 0;
         if (cnt > 0) {
           const i64 cbase = (i64)t * TILE_W;
+          int any = 0;
 #pragma unroll
           for (int c = 0; c < HCH; ++c) {
             if (c < nch) {
               const int nvalid = (int)min((i64)WG, nA - (i64)c * WG);
-              tile_chunk<WG>(ok[c] ? nxt[c] - cur[c] : 0, b0[c] + cur[c], av[c], cbase, bcol,
-                             bval, s_vals, s_bits, s_bs, s_ex, s_av, s_scr, nvalid);
+              any += tile_chunk<WG>(ok[c] ? nxt[c] - cur[c] : 0, b0[c] + cur[c], av[c], cbase,
+                                    bcol, bval, s_vals, s_bits, s_bs, s_ex, s_av, s_scr, nvalid);
             }
           }
-          tile_emit<WG>(out, cbase, s_vals, s_bits, s_scr, ccol, cval);
-          out += cnt;
+          if (any > 0) {
+            const int n = tile_emit<WG>(out, cbase, capacity - (out - base), s_vals, s_bits,
+                                        s_pfx, s_scr, ccol, cval);
+            if (n < 0) over = true; else out += n;
+          }
         }
 #pragma unroll
         for (int c = 0; c < HCH; ++c) {
@@ -596,10 +696,11 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
       }
     } else {
       // Very long A rows: slab boundaries are read per tile and chunk.
-      for (int t = 0; t < nt; ++t) {
-        const int cnt = t < TC_LDS ? s_tc[t] : tc[t];
+      for (int t = 0; t < nt && !over; ++t) {
+        const int cnt = LEARN ? 1 : (t < TC_LDS ? s_tc[t] : tc[t]);
         if (cnt == 0) continue;
         const i64 cbase = (i64)t * TILE_W;
+        int any = 0;
         for (i64 e0 = a0; e0 < a1; e0 += WG) {
           const i64 e = e0 + l;
           int len = 0;
@@ -613,11 +714,20 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
             len = s1 - s0;
             a = aval[e];
           }
-          tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_bits, s_bs, s_ex, s_av,
-                         s_scr, (int)min((i64)WG, a1 - e0));
+          any += tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_bits, s_bs, s_ex,
+                                s_av, s_scr, (int)min((i64)WG, a1 - e0));
         }
-        tile_emit<WG>(out, cbase, s_vals, s_bits, s_scr, ccol, cval);
-        out += cnt;
+        if (any > 0) {
+          const int n = tile_emit<WG>(out, cbase, capacity - (out - base), s_vals, s_bits, s_pfx,
+                                      s_scr, ccol, cval);
+          if (n < 0) over = true; else out += n;
+        }
+      }
+    }
+    if constexpr (LEARN) {
+      if (l == 0) {
+        nnz[row] = over ? 0 : out - base;
+        if (over) ovf[row] = 1;
       }
     }
     __syncthreads();  // s_tc is reloaded for the next row
@@ -860,9 +970,10 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 4);
     {
       ProfScope ps("num_heavy", s);
-      hipLaunchKernelGGL(k_num_heavy<BATCH>, dim3(grid), dim3(1024), 0, s, heavy.list.p + heavy.off,
-                         heavy.n, heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val,
-                         B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
+      hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(1024), 0, s,
+                         heavy.list.p + heavy.off, heavy.n, heavy.nt, heavy.tilecnt.p, key,
+                         tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp,
+                         C.col, C.val, nullptr, nullptr, nullptr);
       RT_HIP(hipGetLastError());
     }
     HashArgs a = make_args(A, B, bbase);
@@ -970,6 +1081,40 @@ void learned_numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i6
                           i32* ovf, const Tuning& tu, hipStream_t s) {
   if (bbase) learned_impl<true>(A, B, bbase, key, cap, tmp_off, tcol, tval, nnz, ovf, tu, s);
   else learned_impl<false>(A, B, nullptr, key, cap, tmp_off, tcol, tval, nnz, ovf, tu, s);
+}
+
+void learned_heavy_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                        const i64* cap, const i64* tmp_off, i32* tcol, float* tval, i64* nnz,
+                        i32* ovf, hipStream_t s) {
+  const i64 m = A.rows;
+  DBuf<i32> list(std::max<i64>(m, 1));
+  int cnt[NBIN], off[NBIN + 1];
+  const i64 zero[5] = {0, 0, 0, 0, 0};  // every row with key > 0 lands in bin 5
+  bin_rows(key, m, LLONG_MAX, zero, list.p, cnt, off, s);
+  if (cnt[5] == 0) return;
+  const i32 nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
+  DBuf<i32> slab((size_t)B.rows * (nt + 1));
+  {
+    const i64 nthreads = B.rows * (i64)(nt + 1);
+    ProfScope ps("slab_ptrs", s);
+    hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
+                       B.rows, nt, B.rp, B.col, slab.p);
+    RT_HIP(hipGetLastError());
+  }
+  const unsigned grid = (unsigned)std::min<i64>(cnt[5], (i64)device().cus * 4);
+  {
+    ProfScope ps("learn_heavy", s);
+    if (bbase)
+      hipLaunchKernelGGL((k_num_heavy<true, true>), dim3(grid), dim3(1024), 0, s, list.p + off[5],
+                         cnt[5], nt, nullptr, key, (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val,
+                         bbase, slab.p, tmp_off, tcol, tval, cap, nnz, ovf);
+    else
+      hipLaunchKernelGGL((k_num_heavy<false, true>), dim3(grid), dim3(1024), 0, s,
+                         list.p + off[5], cnt[5], nt, nullptr, key, (i64)0, A.rp, A.col, A.val,
+                         B.rp, B.col, B.val, nullptr, slab.p, tmp_off, tcol, tval, cap, nnz, ovf);
+    RT_HIP(hipGetLastError());
+  }
+  RT_HIP(hipStreamSynchronize(s));  // list and slab are released at scope exit
 }
 
 void alloc_c(CsrOut& C, const i64* nnz, i64 m, i64 cols, hipStream_t s) {

@@ -67,6 +67,11 @@ void numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
 void learned_numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
                           const i64* cap, const i64* tmp_off, i32* tcol, float* tval, i64* nnz,
                           i32* ovf, const Tuning& tu, hipStream_t s);
+// Long rows (key > 0) sized by the predictor: dense-tile kernel writing at
+// tmp_off with capacity cap; nnz or ovf set per row.
+void learned_heavy_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
+                        const i64* cap, const i64* tmp_off, i32* tcol, float* tval, i64* nnz,
+                        i32* ovf, hipStream_t s);
 // C.rp = exclusive scan of nnz; allocates C.col / C.val.
 void alloc_c(CsrOut& C, const i64* nnz, i64 m, i64 cols, hipStream_t s);
 

@@ -7,8 +7,8 @@ symbolic pass on the seeded synthetic matrices of the first two workloads:
   * A*A, A 4,096 x 4,096 with 8 uniform columns per row, seed 0;
   * A*A, A R-MAT scale 20, edge factor 16, .57/.19/.19/.05, seed 1
     (plus R-MAT scales 16 and 18 with seeds 11 and 12 of the same family).
-Only rows the predictor sizes are kept: rows with products and
-min(products, cols) <= 4096 (longer rows always take the exact path).
+Every row with at least one product is kept: short rows are sized for the
+LDS-hash tables, long rows for the dense-tile kernel's output buffer.
 Training runs on the GPU with the library's own forward/backward/Adam
 kernels (rt_mlp_train).  Writes weights/rownnz_mlp.bin (loaded by the
 library on first use of the learned method) and weights/rownnz_mlp.json.
@@ -43,7 +43,7 @@ def collect(A, B):
     feat, nnz = rt.row_features(A, B)
     P = row_products(A, B)
     U = np.minimum(P, B.shape[1])
-    keep = (P > 0) & (U <= TH)
+    keep = P > 0
     return feat[keep], np.log2(1.0 + nnz[keep]).astype(np.float32)
 
 
