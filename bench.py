@@ -103,8 +103,135 @@ def cpu_baseline(A, P, threads, target_s=15.0):
                       f"FP32 Gustavson with sorted CSR rows, computed and discarded, {t:.1f} s"}
 
 
+def timed_calls(fn, steps, warmup):
+    """Run fn() warmup + steps times; returns ms per timed call (wall) and the
+    last result.  fn returns (result, free_callable)."""
+    import rtspmspm_amd as rt
+    for _ in range(warmup):
+        r, free = fn()
+        free()
+    rt.synchronize()
+    t0 = time.perf_counter()
+    last = None
+    for s in range(steps):
+        r, free = fn()
+        if s + 1 < steps:
+            free()
+        else:
+            last = (r, free)
+    rt.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, last
+
+
+def run_workload(args):
+    """The workloads other than the headline one: one JSON line with plain and
+    learned (and ray for uniform4k, sequential-vs-batched for batch2048)."""
+    import rtspmspm_amd as rt
+    from rtspmspm_amd import workloads as W
+    out = {"workload": args.workload, "steps": args.steps, "warmup": args.warmup}
+    t0 = time.perf_counter()
+
+    def one(dA, dB, mode):
+        def f():
+            C, info = rt.spmspm(dA, dB, mode=mode)
+            return (C, info), C.free
+        return f
+
+    if args.workload in ("uniform4k", "rect"):
+        if args.workload == "uniform4k":
+            A = W.uniform4k()
+            B = A
+            desc = "A*A, A 4096 x 4096, 8 uniform columns per row, seed 0"
+        else:
+            A, B = W.rect()
+            desc = "A 500000 x 2000000 lognormal rows * B 2000000 x 500000 4 per row, seed 3"
+        P = int(W.row_products(A, B).sum())
+        dA, dB = rt.to_device(A), rt.to_device(B)
+        log(f"{args.workload}: generated in {time.perf_counter() - t0:.1f} s, products {P}")
+        modes = ["plain", "learned"] + (["ray"] if args.workload == "uniform4k" else [])
+        for mode in modes:
+            ms, ((C, info), free) = timed_calls(one(dA, dB, mode), args.steps, args.warmup)
+            nnz = C.nnz
+            free()
+            out[mode] = {"ms": round(ms, 4), "gflops": round(2 * P / ms / 1e6, 3),
+                         "overflow_rows": info["overflow_rows"],
+                         "predicted_rows": info["predicted_rows"]}
+            log(f"  {mode}: {ms:.3f} ms, {2 * P / ms / 1e6:.2f} GFLOP/s, nnz(C) {nnz}")
+        bytes_alg = W.csr_bytes(A.shape[0], A.nnz) + (0 if B is A else W.csr_bytes(B.shape[0], B.nnz)) \
+            + W.csr_bytes(A.shape[0], nnz)
+        out.update(config=desc, products=P, nnz_C=nnz, bytes_alg=bytes_alg)
+    elif args.workload == "galerkin":
+        A, Pm, R = W.galerkin(args.grid)
+        dA, dP, dR = rt.to_device(A), rt.to_device(Pm), rt.to_device(R)
+        P1 = int(W.row_products(A, Pm).sum())
+        log(f"galerkin {args.grid}^3: generated in {time.perf_counter() - t0:.1f} s")
+        for mode in ("plain", "learned"):
+            def chain():
+                AP, i1 = rt.spmspm(dA, dP, mode=mode)
+                C, i2 = rt.spmspm(dR, AP, mode=mode)
+                nnz_ap = AP.nnz
+                AP.free()
+                return (C, nnz_ap, i1, i2), C.free
+            ms, ((C, nnz_ap, i1, i2), free) = timed_calls(chain, args.steps, args.warmup)
+            nnz_c = C.nnz
+            free()
+            P2 = i2["products"]
+            out[mode] = {"ms": round(ms, 4), "gflops": round(2 * (P1 + P2) / ms / 1e6, 3),
+                         "overflow_rows": i1["overflow_rows"] + i2["overflow_rows"]}
+            log(f"  {mode}: {ms:.3f} ms, {2 * (P1 + P2) / ms / 1e6:.2f} GFLOP/s, "
+                f"nnz(AP) {nnz_ap}, nnz(C) {nnz_c}")
+        bytes_alg = (W.csr_bytes(A.shape[0], A.nnz) + W.csr_bytes(Pm.shape[0], Pm.nnz)
+                     + 2 * W.csr_bytes(A.shape[0], nnz_ap) + W.csr_bytes(R.shape[0], R.nnz)
+                     + W.csr_bytes(R.shape[0], nnz_c))
+        out.update(config=f"R*(A*P), 27-point stencil on {args.grid}^3, 2x2x2 aggregation",
+                   products=P1 + P2, nnz_AP=nnz_ap, nnz_C=nnz_c, bytes_alg=bytes_alg)
+    elif args.workload == "batch2048":
+        ar, br, bc, SA, SB = W.batch(args.count)
+        dSA, dSB = rt.to_device(SA), rt.to_device(SB)
+        pairs = W.split_pairs(ar, br, bc, SA, SB)
+        dpairs = [(rt.to_device(a), rt.to_device(b)) for a, b in pairs]
+        log(f"batch {args.count}: generated in {time.perf_counter() - t0:.1f} s, "
+            f"rows {int(ar.sum())}, nnz(A) {SA.nnz}")
+        for mode in ("plain", "learned"):
+            def batched():
+                C, info = rt.spmspm_batched(ar, br, bc, dSA, dSB, mode=mode)
+                return (C, info), C.free
+            ms, ((C, info), free) = timed_calls(batched, args.steps, args.warmup)
+            nnz_c, P = C.nnz, info["products"]
+            free()
+            out["batched_" + mode] = {"ms": round(ms, 3), "gflops": round(2 * P / ms / 1e6, 3),
+                                      "overflow_rows": info["overflow_rows"]}
+            log(f"  batched {mode}: {ms:.2f} ms, {2 * P / ms / 1e6:.1f} GFLOP/s, nnz(C) {nnz_c}")
+
+        def sequential():
+            res = [rt.spmspm(a, b, mode="plain")[0] for a, b in dpairs]
+            return res, (lambda: [c.free() for c in res])
+        ms_seq, (res, free) = timed_calls(sequential, max(1, args.steps // 2), 1)
+        seq_nnz = sum(c.nnz for c in res)
+        free()
+        assert seq_nnz == nnz_c, "batched and sequential results differ in size"
+        out["sequential_plain"] = {"ms": round(ms_seq, 3), "gflops": round(2 * P / ms_seq / 1e6, 3)}
+        out["batched_speedup_vs_sequential"] = round(ms_seq / out["batched_learned"]["ms"], 3)
+        log(f"  sequential plain ({args.count} launches): {ms_seq:.2f} ms -> batched learned "
+            f"{out['batched_speedup_vs_sequential']:.2f}x")
+        bytes_alg = (W.csr_bytes(SA.shape[0], SA.nnz) + W.csr_bytes(SB.shape[0], SB.nnz)
+                     + W.csr_bytes(SA.shape[0], nnz_c))
+        out.update(config=f"{args.count} pairs, n log-uniform [256, 65536], Poisson(U(4,32)), seed 4",
+                   products=P, nnz_C=nnz_c, bytes_alg=bytes_alg)
+    for k in ("plain", "learned", "ray", "batched_plain", "batched_learned"):
+        if k in out:
+            out[k]["roofline_frac"] = round(out["bytes_alg"] / out[k]["ms"] / 1e6 / HBM_PEAK_GBS, 6)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="rmat20",
+                    choices=["rmat20", "uniform4k", "galerkin", "rect", "batch2048"],
+                    help="rmat20 (default) prints the benchmark's contract line; the others "
+                         "print one JSON line of per-method timings")
+    ap.add_argument("--grid", type=int, default=128, help="galerkin grid size")
+    ap.add_argument("--count", type=int, default=2048, help="batch2048 pair count")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
@@ -131,6 +258,9 @@ def main():
         sys.exit(2)
     if "RANK" in os.environ and int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py is single-GPU")
+    if args.workload != "rmat20":
+        run_workload(args)
+        return
 
     import rtspmspm_amd as rt
     dev = rt.device_info()

@@ -292,7 +292,8 @@ __global__ __launch_bounds__(256) void k_merge_overflow(i64 m, const i64* __rest
   }
 }
 
-// Copy the rows the learned pass completed from the temporary buffer into C.
+// Copy the rows the learned pass completed from the temporary buffer into C:
+// one wave per row, four coalesced 256-byte loads in flight per array.
 __global__ __launch_bounds__(256) void k_copy_rows(i64 m, const i64* __restrict__ cap,
                                                    const i32* __restrict__ ovf,
                                                    const i64* __restrict__ tmp_off,
@@ -301,14 +302,30 @@ __global__ __launch_bounds__(256) void k_copy_rows(i64 m, const i64* __restrict_
                                                    const i64* __restrict__ crp,
                                                    i32* __restrict__ ccol,
                                                    float* __restrict__ cval) {
-  constexpr int G = 16;
-  const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / G;
-  const int l = threadIdx.x & (G - 1);
+  constexpr int U = 4;
+  const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / WAVE;
+  const int l = threadIdx.x & (WAVE - 1);
   if (row >= m || cap[row] == 0 || ovf[row]) return;
   const i64 src = tmp_off[row], dst = crp[row], n = crp[row + 1] - dst;
-  for (i64 q = l; q < n; q += G) {
-    ccol[dst + q] = tcol[src + q];
-    cval[dst + q] = tval[src + q];
+  for (i64 q0 = l; q0 < n; q0 += U * WAVE) {
+    i32 c[U];
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 q = q0 + u * WAVE;
+      if (q < n) {
+        c[u] = tcol[src + q];
+        v[u] = tval[src + q];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const i64 q = q0 + u * WAVE;
+      if (q < n) {
+        ccol[dst + q] = c[u];
+        cval[dst + q] = v[u];
+      }
+    }
   }
 }
 
@@ -473,7 +490,7 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   if (m > 0) {
     {
       ProfScope ps("copy_rows", s);
-      hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * 16 + 255) / 256)), dim3(256), 0, s, m,
+      hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * WAVE + 255) / 256)), dim3(256), 0, s, m,
                          cap.p, ovf.p, tmp_off.p, tcol.p, tval.p, C.rp, C.col, C.val);
       RT_HIP(hipGetLastError());
     }

@@ -1,0 +1,54 @@
+"""GPU parity of the benchmark workloads at reduced sizes: the Galerkin chain
+R*(A*P), the rectangular product, and the batch (plain and learned)."""
+import numpy as np
+import pytest
+
+import oracle
+import rtspmspm_amd as rt
+from rtspmspm_amd import workloads as W
+from tests.helpers import check_parity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("grid", [16, 32])
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_galerkin_chain(grid, mode):
+    A, P, R = W.galerkin(grid)
+    AP, _ = rt.spmspm(A, P, mode=mode)
+    # closed forms: per dimension the fine stencil reaches 2n-2 aggregate
+    # positions of A*P in total, the coarse operator spans 3(n/2)-2
+    assert AP.nnz == (2 * grid - 2) ** 3
+    check_parity(AP, oracle.spgemm_f64(A, P))
+    C, _ = rt.spmspm(R, AP, mode=mode)
+    assert C.shape == ((grid // 2) ** 3, (grid // 2) ** 3)
+    assert C.nnz == (3 * (grid // 2) - 2) ** 3
+    check_parity(C, oracle.spgemm_f64(R, AP))
+    # every row of A sums to 26 - 26*(1 +- 0.01): the coarse rows sum near 0
+    rows = C.to_scipy().sum(axis=1)
+    assert np.all(np.abs(rows) < 0.02 * 64 * 27)
+
+
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_rectangular_lognormal(mode):
+    A, B = W.rect(rows=30_000, inner=120_000, seed=3)
+    C, info = rt.spmspm(A, B, mode=mode)
+    assert C.shape == (30_000, 30_000)
+    assert info["products"] == int(W.row_products(A, B).sum())
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
+def test_batch_learned_equals_plain_and_oracle():
+    ar, br, bc, SA, SB = W.batch(count=40, nmin=256, nmax=4096, seed=4)
+    dA, dB = rt.to_device(SA), rt.to_device(SB)
+    Cp = rt.spmspm_batched(ar, br, bc, dA, dB, mode="plain")[0].to_host()
+    Cl, info = rt.spmspm_batched(ar, br, bc, dA, dB, mode="learned")
+    Cl = Cl.to_host()
+    assert np.array_equal(Cp.row_ptr, Cl.row_ptr) and np.array_equal(Cp.col, Cl.col)
+    assert info["predicted_rows"] > 0
+    a0 = 0
+    for (Ap, Bp), n in zip(W.split_pairs(ar, br, bc, SA, SB), ar):
+        s, e = Cl.row_ptr[a0], Cl.row_ptr[a0 + n]
+        Cq = rt.CSR((int(n), Bp.shape[1]), Cl.row_ptr[a0:a0 + n + 1] - s, Cl.col[s:e], Cl.val[s:e])
+        check_parity(Cq, oracle.spgemm_f64(Ap, Bp))
+        a0 += int(n)
