@@ -28,6 +28,19 @@ struct Thr {
   i64 t[5];
 };
 
+// A list of rows for a grid-stride kernel: either (list, n) known on the
+// host, or bin `bin` of a device-side binning (count dcount[bin], rows from
+// list[doff[bin]]), so that launches need no host round trip.
+struct RowSet {
+  const i32* list;
+  i32 n;
+  const i32* dcount;
+  const i32* doff;
+  int bin;
+  __device__ __forceinline__ i32 count() const { return dcount ? dcount[bin] : n; }
+  __device__ __forceinline__ const i32* rows() const { return dcount ? list + doff[bin] : list; }
+};
+
 // ---------------------------------------------------------------------------
 // Row analysis: P_i = sum over A(i,k) of nnz(B(k,:)).  16 lanes per row.
 // totals[0] += sum P_i, totals[1] = max P_i.
@@ -149,6 +162,11 @@ struct HashArgs {
   float* cval;
   const i64* cap;
   i32* ovf;
+  // When dcount is set the kernel is persistent: its rows are the bin `bin`
+  // of a device-side binning (count dcount[bin], first row list[doff[bin]]).
+  const i32* dcount;
+  const i32* doff;
+  int bin;
 };
 
 template <int G, int T, int WG, int MODE, bool BATCH>
@@ -166,8 +184,8 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_cnt[MODE == MODE_LEARN ? R : 1];
   __shared__ i32 s_ovf[MODE == MODE_LEARN ? R : 1];
-  const i32* __restrict__ list = a.list;
-  const i32 nlist = a.nlist;
+  const i32 nlist = a.dcount ? a.dcount[a.bin] : a.nlist;
+  const i32* __restrict__ list = a.dcount ? a.list + a.doff[a.bin] : a.list;
   const i64* __restrict__ arp = a.arp;
   const i32* __restrict__ acol = a.acol;
   const float* __restrict__ aval = a.aval;
@@ -180,7 +198,13 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   i32* keys = s_keys + g * T;
   float* vals = NUMERIC ? s_vals + g * T : nullptr;
   const int gb = g * G;
-  const i64 ridx = (i64)blockIdx.x * R + g;
+  const i64 ngroups = ((i64)nlist + R - 1) / R;
+  // Row groups are taken grid-stride (a single pass when the grid covers the
+  // bin).  The loop is uniform over the workgroup; in block mode the tables
+  // are shared by the whole workgroup, hence the barrier between groups.
+  for (i64 gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+  [&]() {
+  const i64 ridx = gi * R + g;
   const bool active = ridx < nlist;
 
   for (int s = l; s < T; s += G) {
@@ -355,6 +379,9 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
       }
     }
   }
+  }();
+  group_sync<G>();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -364,11 +391,13 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
 // tile of the row, which positions the dense-tile numeric kernel's output.
 template <bool BATCH>
 __global__ __launch_bounds__(1024) void k_sym_bitmap(
-    const i32* __restrict__ list, i32 nlist, const i64* __restrict__ arp,
+    RowSet rs, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const i64* __restrict__ brp,
     const i32* __restrict__ bcol, const i64* __restrict__ bbase, i64 ncols, i32 nt,
-    i32* __restrict__ tilecnt, i64* __restrict__ out_nnz) {
+    i32* __restrict__ tilecnt, i64* __restrict__ out_nnz, u32* __restrict__ rowbits) {
   constexpr int WG = 1024;
+  const i32 nlist = rs.count();
+  const i32* __restrict__ list = rs.rows();
   __shared__ u32 s_bits[WIN_W / 32];
   __shared__ i64 s_bs[WG];
   __shared__ i32 s_ex[WG];
@@ -427,6 +456,10 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
         __syncthreads();
       }
       const int ntw = (int)((wcols + TILE_W - 1) / TILE_W);
+      if (rowbits) {  // keep the row's column set (padded to whole tiles) for the numeric pass
+        u32* dst = rowbits + it * ((i64)nt * (TILE_W / 32)) + win / 32;
+        for (int d = l; d < ntw * (TILE_W / 32); d += WG) dst[d] = d < nd ? s_bits[d] : 0u;
+      }
       i64 wsum = 0;
       for (int t = w; t < ntw; t += WG / WAVE) {
         int c = 0;
@@ -608,7 +641,7 @@ __device__ __forceinline__ int tile_emit(i64 out, i64 cbase, i64 limit, float* s
 //   products is skipped.
 template <bool BATCH, bool LEARN>
 __global__ __launch_bounds__(1024) void k_num_heavy(
-    const i32* __restrict__ list, i32 nlist, i32 nt, const i32* __restrict__ tilecnt,
+    RowSet rs, i32 nt, const i32* __restrict__ tilecnt,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const float* __restrict__ aval,
     const i64* __restrict__ brp, const i32* __restrict__ bcol,
@@ -617,6 +650,8 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
     float* __restrict__ cval, const i64* __restrict__ cap, i64* __restrict__ nnz,
     i32* __restrict__ ovf) {
   constexpr int WG = 1024;
+  const i32 nlist = rs.count();
+  const i32* __restrict__ list = rs.rows();
   __shared__ float s_vals[TILE_W];
   __shared__ u32 s_bits[TILE_W / 32];
   __shared__ i32 s_pfx[TILE_W / 32];
@@ -729,6 +764,211 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
         nnz[row] = over ? 0 : out - base;
         if (over) ovf[row] = 1;
       }
+    }
+    __syncthreads();  // s_tc is reloaded for the next row
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Segmented long-row numeric kernel (exact method, row bitmaps available).
+//
+// The symbolic pass stored each long row's column set as a bitmap.  With it
+// the row is processed in few steps: a run of up to SEG_TILES consecutive
+// tiles holding at most SEG_CAP entries is one "compressed" step -- its
+// bitmap words and their popcount prefix are staged in LDS, every product is
+// added at its rank (prefix + popcount of the lower bits of its word), so the
+// accumulators already sit in column order and are written out directly.  A
+// tile holding more than SEG_CAP entries is one "dense" step (the tile
+// accumulator of k_num_heavy).  One workgroup per row, rows grid-stride.
+constexpr int SEG_TILES = 8;
+constexpr int SEG_WORDS = SEG_TILES * (TILE_W / 32);  // 8192 bitmap words
+constexpr int SEG_CAP = 16384;                        // entries per compressed step
+// LDS map (bytes): [0, 128K) dense accumulators; in compressed steps
+// [0, 64K) accumulators, [64K, 96K) bitmap words, [96K, 112K) u16 prefix.
+// [128K, 132K) dense presence bits, [132K, 136K) dense word prefix.
+constexpr int HSEG_LDS = 136 * 1024;
+
+// Products of one chunk restricted to the segment's column range, added at
+// their rank.  Two block-wide barriers; returns the product count.
+template <int WG>
+__device__ __forceinline__ int seg_chunk(int len, i64 start, float av, i64 cbase,
+                                         const i32* __restrict__ bcol,
+                                         const float* __restrict__ bval, float* s_vals,
+                                         const u32* s_cbits, const uint16_t* s_cpfx, i64* s_bs,
+                                         i32* s_ex, float* s_av, i32* s_scr, int nvalid) {
+  const int l = threadIdx.x;
+  int tot;
+  const int inc = block_scan_incl<WG>(len, s_scr, &tot);
+  if (tot == 0) {
+    __syncthreads();
+    return 0;
+  }
+  s_ex[l] = inc - len;
+  s_bs[l] = start;
+  s_av[l] = av;
+  __syncthreads();
+  int lo = 0;
+  for (int p0 = l; p0 < tot; p0 += UNR * WG) {
+    i64 idx[UNR];
+    float a[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int p = p0 + u * WG;
+      idx[u] = -1;
+      a[u] = 0.f;
+      if (p < tot) {
+        lo = advance_entry(s_ex, lo, nvalid, p);
+        idx[u] = s_bs[lo] + (p - s_ex[lo]);
+        a[u] = s_av[lo];
+      }
+    }
+    i32 c[UNR];
+    float v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (idx[u] >= 0) {
+        c[u] = bcol[idx[u]];
+        v[u] = bval[idx[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (idx[u] >= 0) {
+        const int cl = (int)(c[u] - cbase);
+        const int w = cl >> 5;
+        const int rank = s_cpfx[w] + __popc(s_cbits[w] & ((1u << (cl & 31)) - 1u));
+        lds_add_f32(&s_vals[rank], a[u] * v[u]);
+      }
+  }
+  __syncthreads();
+  return tot;
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(1024) void k_num_heavy_seg(
+    RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
+    const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
+    const i32* __restrict__ acol, const float* __restrict__ aval,
+    const i64* __restrict__ brp, const i32* __restrict__ bcol,
+    const float* __restrict__ bval, const i64* __restrict__ bbase,
+    const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
+    float* __restrict__ cval) {
+  constexpr int WG = 1024;
+  constexpr int WPT = SEG_WORDS / WG;  // bitmap words staged per thread
+  static_assert(TILE_W / 32 == WG && WPT == 8, "layout assumes 1024 threads");
+  __shared__ __align__(16) unsigned char s_raw[HSEG_LDS];
+  __shared__ i64 s_bs[WG];
+  __shared__ i32 s_ex[WG];
+  __shared__ float s_av[WG];
+  __shared__ i32 s_scr[WG / WAVE];
+  __shared__ i32 s_tc[TC_LDS];
+  float* s_vals = reinterpret_cast<float*>(s_raw);
+  u32* s_cbits = reinterpret_cast<u32*>(s_raw + 64 * 1024);
+  uint16_t* s_cpfx = reinterpret_cast<uint16_t*>(s_raw + 96 * 1024);
+  u32* s_dbits = reinterpret_cast<u32*>(s_raw + 128 * 1024);
+  i32* s_dpfx = reinterpret_cast<i32*>(s_raw + 132 * 1024);
+  const i32 nlist = rs.count();
+  const i32* __restrict__ list = rs.rows();
+  const int l = threadIdx.x;
+  const i64 row_words = (i64)nt * (TILE_W / 32);
+  for (int d = l; d < TILE_W; d += WG) s_vals[d] = 0.f;
+  s_dbits[l] = 0;
+  __syncthreads();
+  for (i64 h = blockIdx.x; h < nlist; h += gridDim.x) {
+    const i32 row = list[h];
+    if (key[row] <= th) continue;
+    const i32* tc = tilecnt + h * nt;
+    for (int t = l; t < min(nt, TC_LDS); t += WG) s_tc[t] = tc[t];
+    const i64 a0 = arp[row], a1 = arp[row + 1];
+    const i64 bb = BATCH ? bbase[row] : 0;
+    const u32* bits = rowbits + h * row_words;
+    i64 out = crp[row];
+    __syncthreads();
+    int t = 0;
+    while (t < nt) {
+      const int c0 = t < TC_LDS ? s_tc[t] : tc[t];
+      if (c0 == 0) {
+        ++t;
+        continue;
+      }
+      int t1 = t + 1, n = c0;
+      const bool dense = c0 > SEG_CAP;
+      if (!dense) {
+        while (t1 < nt && t1 - t < SEG_TILES) {
+          const int ct = t1 < TC_LDS ? s_tc[t1] : tc[t1];
+          if (ct > SEG_CAP || n + ct > SEG_CAP) break;
+          n += ct;
+          ++t1;
+        }
+        // Stage the segment's bitmap words and their prefix.
+        const int nw = (t1 - t) * (TILE_W / 32);
+        const u32* src = bits + (i64)t * (TILE_W / 32);
+        u32 wv[WPT];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+          const int w = l * WPT + j;
+          wv[j] = w < nw ? src[w] : 0u;
+          cnt += __popc(wv[j]);
+        }
+        int tot;
+        const int inc = block_scan_incl<WG>(cnt, s_scr, &tot);
+        int run = inc - cnt;
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+          s_cbits[l * WPT + j] = wv[j];
+          s_cpfx[l * WPT + j] = (uint16_t)run;
+          run += __popc(wv[j]);
+        }
+        __syncthreads();
+      }
+      const i64 cbase = (i64)t * TILE_W;
+      // Products of every A entry inside tiles [t, t1).
+      for (i64 e0 = a0; e0 < a1; e0 += WG) {
+        const i64 e = e0 + l;
+        int len = 0;
+        i64 start = 0;
+        float a = 0.f;
+        if (e < a1) {
+          const i64 k = (i64)acol[e] + bb;
+          const i32* sk = slab + k * (nt + 1);
+          const i32 s0 = sk[t], s1 = sk[t1];
+          start = brp[k] + s0;
+          len = s1 - s0;
+          a = aval[e];
+        }
+        const int nvalid = (int)min((i64)WG, a1 - e0);
+        if (dense)
+          tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_dbits, s_bs, s_ex, s_av,
+                         s_scr, nvalid);
+        else
+          seg_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_cbits, s_cpfx, s_bs, s_ex,
+                        s_av, s_scr, nvalid);
+      }
+      if (dense) {
+        tile_emit<WG>(out, cbase, LLONG_MAX, s_vals, s_dbits, s_dpfx, s_scr, ccol, cval);
+      } else {
+        // Accumulators are in column order: entry q is the q-th set bit.
+        const int nw = (t1 - t) * (TILE_W / 32);
+        int w = 0;
+        for (int q = l; q < n; q += WG) {
+          int hi = nw - 1;
+          while (w < hi) {
+            const int mid = (w + hi + 1) >> 1;
+            if ((int)s_cpfx[mid] <= q) w = mid; else hi = mid - 1;
+          }
+          const int bit = select_bit(s_cbits[w], q - (int)s_cpfx[w]);
+          ccol[out + q] = (i32)(cbase + w * 32 + bit);
+          cval[out + q] = s_vals[q];
+          s_vals[q] = 0.f;
+        }
+        __syncthreads();
+        // The next step may be dense: its accumulators span the region the
+        // bitmap and prefix used.
+        for (int d = SEG_CAP + l; d < TILE_W; d += WG) s_vals[d] = 0.f;
+        __syncthreads();
+      }
+      out += n;
+      t = t1;
     }
     __syncthreads();  // s_tc is reloaded for the next row
   }
@@ -941,11 +1181,18 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
   heavy.nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
   if (heavy.n > 0) {
     heavy.tilecnt.alloc((size_t)heavy.n * heavy.nt);
+    // Keep the long rows' column bitmaps for the segmented numeric kernel
+    // when they fit comfortably in free device memory.
+    const size_t bm_bytes = (size_t)heavy.n * heavy.nt * (TILE_W / 8);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    if (bm_bytes <= free_b / 3 + pool_cached_bytes() / 3)
+      heavy.rowbits.alloc(bm_bytes / sizeof(u32));
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 8);
     ProfScope ps("sym_bitmap", s);
-    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, heavy.list.p + heavy.off,
-                       heavy.n, A.rp, A.col, B.rp, B.col, bbase, B.cols, heavy.nt,
-                       heavy.tilecnt.p, nnz);
+    const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, rs, A.rp, A.col, B.rp,
+                       B.col, bbase, B.cols, heavy.nt, heavy.tilecnt.p, nnz, heavy.rowbits.p);
     RT_HIP(hipGetLastError());
   }
 }
@@ -970,10 +1217,16 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 4);
     {
       ProfScope ps("num_heavy", s);
-      hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(1024), 0, s,
-                         heavy.list.p + heavy.off, heavy.n, heavy.nt, heavy.tilecnt.p, key,
-                         tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp,
-                         C.col, C.val, nullptr, nullptr, nullptr);
+      const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
+      if (heavy.rowbits.p)
+        hipLaunchKernelGGL(k_num_heavy_seg<BATCH>, dim3(grid), dim3(1024), 0, s, rs, heavy.nt,
+                           heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col, A.val,
+                           B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
+      else
+        hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(1024), 0, s, rs,
+                           heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val, B.rp,
+                           B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr,
+                           nullptr);
       RT_HIP(hipGetLastError());
     }
     HashArgs a = make_args(A, B, bbase);
@@ -1105,13 +1358,15 @@ void learned_heavy_pass(const Csr& A, const Csr& B, const i64* bbase, const i64*
   {
     ProfScope ps("learn_heavy", s);
     if (bbase)
-      hipLaunchKernelGGL((k_num_heavy<true, true>), dim3(grid), dim3(1024), 0, s, list.p + off[5],
-                         cnt[5], nt, nullptr, key, (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val,
-                         bbase, slab.p, tmp_off, tcol, tval, cap, nnz, ovf);
+      hipLaunchKernelGGL((k_num_heavy<true, true>), dim3(grid), dim3(1024), 0, s,
+                         RowSet{list.p + off[5], cnt[5], nullptr, nullptr, 0}, nt, nullptr, key,
+                         (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p, tmp_off,
+                         tcol, tval, cap, nnz, ovf);
     else
       hipLaunchKernelGGL((k_num_heavy<false, true>), dim3(grid), dim3(1024), 0, s,
-                         list.p + off[5], cnt[5], nt, nullptr, key, (i64)0, A.rp, A.col, A.val,
-                         B.rp, B.col, B.val, nullptr, slab.p, tmp_off, tcol, tval, cap, nnz, ovf);
+                         RowSet{list.p + off[5], cnt[5], nullptr, nullptr, 0}, nt, nullptr, key,
+                         (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, nullptr, slab.p, tmp_off,
+                         tcol, tval, cap, nnz, ovf);
     RT_HIP(hipGetLastError());
   }
   RT_HIP(hipStreamSynchronize(s));  // list and slab are released at scope exit

@@ -51,6 +51,7 @@ struct HeavyRows {
   DBuf<i32> list;  // all binned rows; the heavy ones start at `off`
   i32 off = 0, n = 0, nt = 1;
   DBuf<i32> tilecnt;  // n x nt
+  DBuf<u32> rowbits;  // n x nt x 1024 words: each long row's column set (may be empty)
 };
 
 // Pipeline pieces.  `key` arrays select and size rows: rows with key 0 are

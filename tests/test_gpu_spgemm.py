@@ -109,6 +109,26 @@ def test_long_rows_uniform(n, k, seed):
     check_parity(C, oracle.spgemm_f64(A, A))
 
 
+def test_long_rows_dense_and_compressed_steps():
+    """Rows of ~39K entries over 40K columns: the first 32768-column tile holds
+    more than 16384 entries (dense step), the rest a compressed step."""
+    A = rt.gen_fixed_normal(200, 40000, 400, 41)
+    B = rt.gen_uniform_k(40000, 400, 42)
+    C, _ = run(A, B)
+    assert np.diff(C.row_ptr).min() > 32768  # tile 0 of every row is dense
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
+def test_long_rows_sparse_over_many_tiles():
+    """~5K entries spread over 31 tiles: compressed steps cut by the 8-tile
+    span limit rather than by their entry count."""
+    A = rt.gen_fixed_normal(100, 5000, 250, 43)
+    B = rt.gen_fixed_normal(5000, 1_000_000, 20, 44)
+    C, info = run(A, B)
+    assert info["products"] == 100 * 250 * 20
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
 def test_mixed_row_lengths_rmat():
     A = rt.gen_rmat(14, 16, seed=5)
     C, _ = run(A, A)
