@@ -83,7 +83,9 @@ typedef struct rt_opts {
   int32_t ray_output;  /* RT_RAY_* (RT_MODE_RAY only)                       */
   int32_t profile;     /* 1: time every kernel launch with hipEvents        */
   int32_t use_tuned;   /* 1: use/record autotuned bin thresholds            */
-  int32_t reserved[3];
+  int32_t graph;       /* plain method: 0 = default (on for batched calls),
+                          1 = capture each stage in a hipGraph, 2 = direct */
+  int32_t reserved[2];
   /* outputs */
   double   ms_total;       /* hipEvent time, inputs resident -> C complete  */
   int64_t  products;       /* intermediate products P                       */

@@ -49,7 +49,8 @@ class rt_csr(ctypes.Structure):
 class rt_opts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("ablation", ctypes.c_int32),
                 ("ray_output", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("use_tuned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+                ("use_tuned", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2),
                 ("ms_total", ctypes.c_double), ("products", ctypes.c_int64),
                 ("overflow_rows", ctypes.c_int64), ("predicted_rows", ctypes.c_int64)]
 
@@ -212,13 +213,17 @@ def to_device(A: CSR) -> DeviceCSR:
     return DeviceCSR(d)
 
 
-def _opts(mode, ablation, ray_output, profile, use_tuned) -> rt_opts:
+GRAPH = {None: 0, True: 1, False: 2}
+
+
+def _opts(mode, ablation, ray_output, profile, use_tuned, graph=None) -> rt_opts:
     o = rt_opts()
     o.mode = MODES[mode]
     o.ablation = ABLATIONS[ablation]
     o.ray_output = RAY_OUTPUTS[ray_output]
     o.profile = 1 if profile else 0
     o.use_tuned = 1 if use_tuned else 0
+    o.graph = GRAPH[graph]
     return o
 
 
@@ -228,11 +233,11 @@ def _info(o: rt_opts) -> dict:
 
 
 def spmspm(A, B, mode: str = "plain", ablation: str = "full", ray_output: str = "auto",
-           profile: bool = False, use_tuned: bool = False):
+           profile: bool = False, use_tuned: bool = False, graph=None):
     """C = A*B.  Host CSR operands give a host CSR; DeviceCSR operands give a
     DeviceCSR.  Returns (C, info) with info = {ms, products, overflow_rows,
     predicted_rows}; ms is the device time from inputs resident to C complete."""
-    o = _opts(mode, ablation, ray_output, profile, use_tuned)
+    o = _opts(mode, ablation, ray_output, profile, use_tuned, graph)
     C = rt_csr()
     if isinstance(A, DeviceCSR):
         if not isinstance(B, DeviceCSR):
@@ -246,12 +251,15 @@ def spmspm(A, B, mode: str = "plain", ablation: str = "full", ray_output: str = 
 
 
 def spmspm_batched(a_rows, b_rows, b_cols, A_stack: DeviceCSR, B_stack: DeviceCSR,
-                   mode: str = "plain", profile: bool = False, use_tuned: bool = False):
-    """Independent products of stacked device operands (see rt_spmspm_batched)."""
+                   mode: str = "plain", profile: bool = False, use_tuned: bool = False,
+                   graph=None):
+    """Independent products of stacked device operands (see rt_spmspm_batched).
+    graph: None = library default (hipGraph stages for the plain method),
+    True / False = force on / off."""
     a_rows = np.ascontiguousarray(a_rows, np.int64)
     b_rows = np.ascontiguousarray(b_rows, np.int64)
     b_cols = np.ascontiguousarray(b_cols, np.int64)
-    o = _opts(mode, "full", "auto", profile, use_tuned)
+    o = _opts(mode, "full", "auto", profile, use_tuned, graph)
     C = rt_csr()
     _check(_L.rt_spmspm_batched(int(a_rows.size), a_rows.ctypes.data_as(_P(_c.c_int64)),
                                 b_rows.ctypes.data_as(_P(_c.c_int64)),

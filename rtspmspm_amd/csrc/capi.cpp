@@ -168,8 +168,14 @@ void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts&
   Prof::enabled = o.profile != 0;
   RT_HIP(hipEventRecord(ev.a, s));
   try {
+    const bool graph = o.graph == 1 || (o.graph == 0 && bt.b_base != nullptr);
     switch (o.mode) {
-      case RT_MODE_PLAIN: spgemm_plain(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st); break;
+      case RT_MODE_PLAIN:
+        if (graph)
+          spgemm_plain_graph(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st);
+        else
+          spgemm_plain(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st);
+        break;
       case RT_MODE_RAY:
         RT_CHECK(bt.b_base == nullptr, RT_E_INVALID, "the ray method takes a single product");
         spgemm_ray(A, B, C, o.ablation, o.ray_output, s, &st);

@@ -101,6 +101,8 @@ struct Prof {
   static void begin(const char* name, hipStream_t s);
   static void end(hipStream_t s);
   static void flush();  // resolve pending event pairs (synchronises)
+  // Adds an already measured duration (e.g. a whole captured graph).
+  static void record(const char* name, double ms);
 };
 
 struct ProfScope {

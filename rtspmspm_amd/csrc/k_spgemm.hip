@@ -130,6 +130,51 @@ __device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo,
   return lo;
 }
 
+// Per-lane walk over a chunk's flat product range for workgroup-wide loops.
+// The chunk's products [0, tot) are cut into one contiguous slice per wave;
+// lane l of a wave takes slice products l, l+64, ..., so each memory
+// instruction of the wave still reads 64 consecutive products.  The lane
+// keeps its current A entry (start of the next entry, B offset, A value) in
+// registers and touches LDS only when it crosses into another entry.
+struct ProdCursor {
+  int p, end, lo, nx;
+  i64 off;  // B index of product p is off + p
+  float a;
+
+  template <int WG>
+  __device__ __forceinline__ void init(int tot, int nvalid, const i32* ex, const i64* bs,
+                                       const float* av) {
+    constexpr int NWV = WG / WAVE;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int per = (tot + NWV - 1) / NWV;
+    const int beg = w * per;
+    end = min(tot, beg + per);
+    p = beg + lane;
+    lo = 0;
+    nx = 0;
+    off = 0;
+    a = 0.f;
+    if (p < end) refresh(nvalid, ex, bs, av);
+  }
+  __device__ __forceinline__ void refresh(int nvalid, const i32* ex, const i64* bs,
+                                          const float* av) {
+    lo = advance_entry(ex, lo, nvalid, p);
+    nx = lo + 1 < nvalid ? ex[lo + 1] : INT_MAX;
+    off = bs[lo] - ex[lo];
+    a = av ? av[lo] : 0.f;
+  }
+  // Next product of this lane: its B index and A value; false when done.
+  __device__ __forceinline__ bool next(int nvalid, const i32* ex, const i64* bs, const float* av,
+                                       i64& idx, float& aval) {
+    if (p >= end) return false;
+    if (p >= nx) refresh(nvalid, ex, bs, av);
+    idx = off + p;
+    aval = a;
+    p += WAVE;
+    return true;
+  }
+};
+
 // LDS-hash row kernel.  A group of G lanes owns one output row and a table of
 // T slots (linear probing, load factor <= 1/2 by binning).  The products of
 // the row are enumerated flat: a chunk of up to G entries A(i,k) is loaded,
@@ -430,18 +475,18 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
         s_bs[l] = bs;
         __syncthreads();
         const int nvalid = (int)min((i64)WG, a1 - e0);
-        int lo = 0;
-        for (int p0 = l; p0 < tot; p0 += UNR * WG) {
+        ProdCursor cur;
+        cur.init<WG>(tot, nvalid, s_ex, s_bs, nullptr);
+        while (true) {
           i64 idx[UNR];
+          bool any = false;
 #pragma unroll
           for (int u = 0; u < UNR; ++u) {
-            const int p = p0 + u * WG;
-            idx[u] = -1;
-            if (p < tot) {
-              lo = advance_entry(s_ex, lo, nvalid, p);
-              idx[u] = s_bs[lo] + (p - s_ex[lo]);
-            }
+            float unused;
+            if (cur.next(nvalid, s_ex, s_bs, nullptr, idx[u], unused)) any = true;
+            else idx[u] = -1;
           }
+          if (!any) break;
           i32 cc[UNR];
 #pragma unroll
           for (int u = 0; u < UNR; ++u)
@@ -537,21 +582,18 @@ __device__ __forceinline__ int tile_chunk(int len, i64 start, float av, i64 cbas
   s_bs[l] = start;
   s_av[l] = av;
   __syncthreads();
-  int lo = 0;
-  for (int p0 = l; p0 < tot; p0 += UNR * WG) {
+  ProdCursor cur;
+  cur.init<WG>(tot, nvalid, s_ex, s_bs, s_av);
+  while (true) {
     i64 idx[UNR];
     float a[UNR];
+    bool any = false;
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const int p = p0 + u * WG;
-      idx[u] = -1;
-      a[u] = 0.f;
-      if (p < tot) {
-        lo = advance_entry(s_ex, lo, nvalid, p);
-        idx[u] = s_bs[lo] + (p - s_ex[lo]);
-        a[u] = s_av[lo];
-      }
+      if (cur.next(nvalid, s_ex, s_bs, s_av, idx[u], a[u])) any = true;
+      else idx[u] = -1;
     }
+    if (!any) break;
     i32 c[UNR];
     float v[UNR];
 #pragma unroll
@@ -784,7 +826,8 @@ constexpr int SEG_TILES = 8;
 constexpr int SEG_WORDS = SEG_TILES * (TILE_W / 32);  // 8192 bitmap words
 constexpr int SEG_CAP = 16384;                        // entries per compressed step
 // LDS map (bytes): [0, 128K) dense accumulators; in compressed steps
-// [0, 64K) accumulators, [64K, 96K) bitmap words, [96K, 112K) u16 prefix.
+// [0, 64K) accumulators and [64K, 128K) one u64 per bitmap word: the
+// word's popcount prefix in the high half, its bits in the low half.
 // [128K, 132K) dense presence bits, [132K, 136K) dense word prefix.
 constexpr int HSEG_LDS = 136 * 1024;
 
@@ -794,7 +837,7 @@ template <int WG>
 __device__ __forceinline__ int seg_chunk(int len, i64 start, float av, i64 cbase,
                                          const i32* __restrict__ bcol,
                                          const float* __restrict__ bval, float* s_vals,
-                                         const u32* s_cbits, const uint16_t* s_cpfx, i64* s_bs,
+                                         const u64* s_cw, i64* s_bs,
                                          i32* s_ex, float* s_av, i32* s_scr, int nvalid) {
   const int l = threadIdx.x;
   int tot;
@@ -807,21 +850,18 @@ __device__ __forceinline__ int seg_chunk(int len, i64 start, float av, i64 cbase
   s_bs[l] = start;
   s_av[l] = av;
   __syncthreads();
-  int lo = 0;
-  for (int p0 = l; p0 < tot; p0 += UNR * WG) {
+  ProdCursor cur;
+  cur.init<WG>(tot, nvalid, s_ex, s_bs, s_av);
+  while (true) {
     i64 idx[UNR];
     float a[UNR];
+    bool any = false;
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      const int p = p0 + u * WG;
-      idx[u] = -1;
-      a[u] = 0.f;
-      if (p < tot) {
-        lo = advance_entry(s_ex, lo, nvalid, p);
-        idx[u] = s_bs[lo] + (p - s_ex[lo]);
-        a[u] = s_av[lo];
-      }
+      if (cur.next(nvalid, s_ex, s_bs, s_av, idx[u], a[u])) any = true;
+      else idx[u] = -1;
     }
+    if (!any) break;
     i32 c[UNR];
     float v[UNR];
 #pragma unroll
@@ -834,8 +874,8 @@ __device__ __forceinline__ int seg_chunk(int len, i64 start, float av, i64 cbase
     for (int u = 0; u < UNR; ++u)
       if (idx[u] >= 0) {
         const int cl = (int)(c[u] - cbase);
-        const int w = cl >> 5;
-        const int rank = s_cpfx[w] + __popc(s_cbits[w] & ((1u << (cl & 31)) - 1u));
+        const u64 wd = s_cw[cl >> 5];  // prefix in the high half, bits in the low half
+        const int rank = (int)(wd >> 32) + __popc((u32)wd & ((1u << (cl & 31)) - 1u));
         lds_add_f32(&s_vals[rank], a[u] * v[u]);
       }
   }
@@ -862,8 +902,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   float* s_vals = reinterpret_cast<float*>(s_raw);
-  u32* s_cbits = reinterpret_cast<u32*>(s_raw + 64 * 1024);
-  uint16_t* s_cpfx = reinterpret_cast<uint16_t*>(s_raw + 96 * 1024);
+  u64* s_cw = reinterpret_cast<u64*>(s_raw + 64 * 1024);
   u32* s_dbits = reinterpret_cast<u32*>(s_raw + 128 * 1024);
   i32* s_dpfx = reinterpret_cast<i32*>(s_raw + 132 * 1024);
   const i32 nlist = rs.count();
@@ -915,8 +954,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
         int run = inc - cnt;
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
-          s_cbits[l * WPT + j] = wv[j];
-          s_cpfx[l * WPT + j] = (uint16_t)run;
+          s_cw[l * WPT + j] = ((u64)(u32)run << 32) | wv[j];
           run += __popc(wv[j]);
         }
         __syncthreads();
@@ -941,7 +979,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
           tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_dbits, s_bs, s_ex, s_av,
                          s_scr, nvalid);
         else
-          seg_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_cbits, s_cpfx, s_bs, s_ex,
+          seg_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_cw, s_bs, s_ex,
                         s_av, s_scr, nvalid);
       }
       if (dense) {
@@ -954,9 +992,10 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
           int hi = nw - 1;
           while (w < hi) {
             const int mid = (w + hi + 1) >> 1;
-            if ((int)s_cpfx[mid] <= q) w = mid; else hi = mid - 1;
+            if ((int)(s_cw[mid] >> 32) <= q) w = mid; else hi = mid - 1;
           }
-          const int bit = select_bit(s_cbits[w], q - (int)s_cpfx[w]);
+          const u64 wd = s_cw[w];
+          const int bit = select_bit((u32)wd, q - (int)(wd >> 32));
           ccol[out + q] = (i32)(cbase + w * 32 + bit);
           cval[out + q] = s_vals[q];
           s_vals[q] = 0.f;
@@ -1393,6 +1432,216 @@ void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   symbolic_pass(A, B, batch.b_base, prod.p, nnz.p, tune, s, heavy);
   alloc_c(C, nnz.p, m, B.cols, s);
   numeric_pass(A, B, batch.b_base, nnz.p, heavy, C, tune, s);
+}
+
+}  // namespace rt
+
+// ---------------------------------------------------------------------------
+// Graph-captured exact method.
+//
+// The same kernels, launched so that a stage needs no host round trip: row
+// bins are counted, offset and filled on the device, every row kernel is
+// persistent (fixed grid, its row count read from device memory), clears
+// are fill kernels, and all scratch is allocated before capture.  Stage A
+// (products, binning, symbolic, scan) and stage B (binning by nnz, long-row
+// and LDS-hash numeric) are each captured into a hipGraph and launched; the
+// only host read between them is C's size, to allocate C.
+namespace rt {
+using namespace dev;
+
+__global__ __launch_bounds__(256) void k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
+}
+__global__ __launch_bounds__(256) void k_fill_i32(i32* __restrict__ p, i64 n, i32 v) {
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
+}
+// off[b] = exclusive prefix of cnt, cursor[b] = off[b].
+__global__ void k_bin_offsets(const i32* __restrict__ cnt, i32* __restrict__ off,
+                              i32* __restrict__ cursor) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    i32 run = 0;
+    for (int b = 0; b < NBIN; ++b) {
+      off[b] = run;
+      cursor[b] = run;
+      run += cnt[b];
+    }
+    off[NBIN] = run;
+  }
+}
+
+namespace {
+
+unsigned fill_grid(i64 n) {
+  return (unsigned)std::max<i64>(1, std::min<i64>((n + 255) / 256, (i64)device().cus * 8));
+}
+
+struct DevBins {
+  DBuf<i32> list, cnt, off, cursor;
+  explicit DevBins(i64 m) : list(std::max<i64>(m, 1)), cnt(NBIN), off(NBIN + 1), cursor(NBIN) {}
+};
+
+void bin_rows_dev(const i64* x, i64 n, i64 cap, const i64 th_in[5], DevBins& b, hipStream_t s) {
+  Thr th;
+  for (int i = 0; i < 5; ++i) th.t[i] = th_in[i];
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.cnt.p, (i64)NBIN, 0);
+  if (n > 0) {
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, b.cnt.p);
+    hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, b.cnt.p, b.off.p, b.cursor.p);
+    hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, b.cursor.p, b.list.p);
+  } else {
+    hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, b.cnt.p, b.off.p, b.cursor.p);
+  }
+  RT_HIP(hipGetLastError());
+}
+
+template <int G, int T, int WG, int MODE, bool BATCH>
+void launch_hash_dev(HashArgs a, const DevBins& b, int bin, i64 max_rows, hipStream_t s) {
+  constexpr int R = WG / G;
+  a.list = b.list.p;
+  a.nlist = 0;
+  a.dcount = b.cnt.p;
+  a.doff = b.off.p;
+  a.bin = bin;
+  const i64 groups = std::max<i64>(1, (max_rows + R - 1) / R);
+  const unsigned grid = (unsigned)std::min<i64>(groups, (i64)device().cus * (2048 / WG));
+  hipLaunchKernelGGL((k_hash<G, T, WG, MODE, BATCH>), dim3(grid), dim3(WG), 0, s, a);
+  RT_HIP(hipGetLastError());
+}
+
+template <int MODE, bool BATCH>
+void launch_hash_bins_dev(const HashArgs& a, const DevBins& b, i64 max_rows, hipStream_t s) {
+  launch_hash_dev<8, 32, 256, MODE, BATCH>(a, b, 0, max_rows, s);
+  launch_hash_dev<16, 128, 256, MODE, BATCH>(a, b, 1, max_rows, s);
+  launch_hash_dev<64, 512, 256, MODE, BATCH>(a, b, 2, max_rows, s);
+  launch_hash_dev<256, 2048, 256, MODE, BATCH>(a, b, 3, max_rows, s);
+  launch_hash_dev<1024, 8192, 1024, MODE, BATCH>(a, b, 4, max_rows, s);
+}
+
+// Captures the launches `body` issues on `s` into a graph, runs it once and
+// returns the device time of the run.
+template <class F>
+double run_captured(hipStream_t s, F&& body) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  RT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t tmp = nullptr;
+    (void)hipStreamEndCapture(s, &tmp);
+    if (tmp) (void)hipGraphDestroy(tmp);
+    throw;
+  }
+  RT_HIP(hipStreamEndCapture(s, &g));
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    RT_HIP(e);
+  }
+  hipEvent_t a, b;
+  RT_HIP(hipEventCreate(&a));
+  RT_HIP(hipEventCreate(&b));
+  RT_HIP(hipEventRecord(a, s));
+  RT_HIP(hipGraphLaunch(ge, s));
+  RT_HIP(hipEventRecord(b, s));
+  RT_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  RT_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipGraphExecDestroy(ge);
+  (void)hipGraphDestroy(g);
+  return ms;
+}
+
+template <bool BATCH>
+void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, const Tuning& tu,
+                      hipStream_t s, RunStats* st) {
+  const i64 m = A.rows, M = std::max<i64>(m, 1);
+  const i32 nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
+  RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
+  RT_CHECK((double)M * nt * 4.0 < 8.0 * (1ull << 30), RT_E_LIMIT,
+           "graph mode: tile-count scratch too large; use the direct launch path");
+  DBuf<i64> prod(M), nnz(M);
+  DBuf<unsigned long long> tot(2);
+  DevBins sb(m), nb(m);
+  DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
+  const i64 nbs = std::max<i64>(1, (m + SCAN_TILE - 1) / SCAN_TILE);
+  DBuf<i64> bsum(nbs);
+  C.rows = m;
+  C.cols = B.cols;
+  C.rp = static_cast<i64*>(dmalloc((m + 1) * sizeof(i64)));
+  HashArgs base = make_args(A, B, bbase);
+  const unsigned heavy_grid = (unsigned)device().cus * 4;
+  double ms_a = 0, ms_b = 0;
+  try {
+    ms_a = run_captured(s, [&] {
+      hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(tot.p),
+                         (i64)2, (i64)0);
+      if (m > 0)
+        hipLaunchKernelGGL(k_row_products<BATCH>, dim3((unsigned)((m * 16 + 255) / 256)),
+                           dim3(256), 0, s, m, A.rp, A.col, B.rp, bbase, prod.p, tot.p);
+      hipLaunchKernelGGL(k_fill_i64, dim3(fill_grid(M)), dim3(256), 0, s, nnz.p, M, (i64)0);
+      bin_rows_dev(prod.p, m, B.cols, tu.sym, sb, s);
+      HashArgs a = base;
+      a.nnz = nnz.p;
+      launch_hash_bins_dev<MODE_SYM, BATCH>(a, sb, m, s);
+      const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
+      hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(heavy_grid * 2), dim3(1024), 0, s, heavy, A.rp,
+                         A.col, B.rp, B.col, bbase, B.cols, nt, tilecnt.p, nnz.p, nullptr);
+      hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
+      hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nbs, C.rp + m);
+      if (m > 0)
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p,
+                           C.rp);
+      RT_HIP(hipGetLastError());
+    });
+    C.nnz = read_scalar(C.rp + m, s);
+    C.col = static_cast<i32*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(i32)));
+    C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
+    ms_b = run_captured(s, [&] {
+      bin_rows_dev(nnz.p, m, LLONG_MAX, tu.num, nb, s);
+      if (B.rows > 0)
+        hipLaunchKernelGGL(k_slab_ptrs,
+                           dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)), dim3(256), 0,
+                           s, B.rows, nt, B.rp, B.col, slab.p);
+      const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
+      hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(heavy_grid), dim3(1024), 0, s, heavy,
+                         nt, tilecnt.p, nnz.p, tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val,
+                         bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr, nullptr);
+      HashArgs a = base;
+      a.crp = C.rp;
+      a.ccol = C.col;
+      a.cval = C.val;
+      launch_hash_bins_dev<MODE_NUM, BATCH>(a, nb, m, s);
+      RT_HIP(hipGetLastError());
+    });
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    throw;
+  }
+  unsigned long long h[2];
+  RT_HIP(hipMemcpyAsync(h, tot.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipStreamSynchronize(s));
+  if (st) {
+    st->products = (i64)h[0];
+    st->max_row_products = (i64)h[1];
+  }
+  if (Prof::enabled) {
+    Prof::record("graph_stage_a", ms_a);
+    Prof::record("graph_stage_b", ms_b);
+  }
+}
+
+}  // namespace
+
+void spgemm_plain_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                        const Tuning& tune, hipStream_t s, RunStats* st) {
+  if (batch.b_base)
+    plain_graph_impl<true>(A, B, batch.b_base, C, tune, s, st);
+  else
+    plain_graph_impl<false>(A, B, nullptr, C, tune, s, st);
 }
 
 }  // namespace rt

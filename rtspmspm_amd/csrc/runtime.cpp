@@ -191,6 +191,13 @@ void Prof::flush() {
   g_pending.clear();
 }
 
+void Prof::record(const char* name, double ms) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  ProfEntry& e = entry(name);
+  e.launches += 1;
+  e.ms += ms;
+}
+
 void prof_reset() {
   Prof::flush();
   std::lock_guard<std::mutex> g(g_prof_mu);

@@ -80,6 +80,11 @@ void alloc_c(CsrOut& C, const i64* nnz, i64 m, i64 cols, hipStream_t s);
 void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                   const Tuning& tune, hipStream_t s, RunStats* st);
 
+// The same method with each stage captured into a hipGraph (persistent row
+// kernels, device-side binning; one host read for C's size).
+void spgemm_plain_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                        const Tuning& tune, hipStream_t s, RunStats* st);
+
 // Faithful one-work-item-per-A-nonzero launch (reference raygen + any-hit).
 void spgemm_ray(const Csr& A, const Csr& B, CsrOut& C, int ablation, int output,
                 hipStream_t s, RunStats* st);

@@ -159,6 +159,26 @@ def test_device_resident_operands():
     check_parity(dC.to_host(), oracle.spgemm_f64(A, A))
 
 
+@pytest.mark.parametrize("make", [lambda: rt.gen_rmat(13, 16, seed=51),
+                                  lambda: rt.gen_uniform_k(20000, 100, 52)])
+def test_graph_captured_stages_equal_direct(make):
+    """Plain method with both stages captured in hipGraphs (persistent row
+    kernels, device-side binning): identical structure, parity values."""
+    A = make()
+    Cg, _ = run(A, A, mode="plain", graph=True)
+    Cd, _ = run(A, A, mode="plain", graph=False)
+    assert np.array_equal(Cg.row_ptr, Cd.row_ptr) and np.array_equal(Cg.col, Cd.col)
+    check_parity(Cg, oracle.spgemm_f64(A, A))
+
+
+def test_batched_graph_and_direct_agree():
+    ar, br, bc, A, B = rt.gen_batch_poisson(32, 64, 5000, 4.0, 32.0, 7)
+    dA, dB = rt.to_device(A), rt.to_device(B)
+    Cg = rt.spmspm_batched(ar, br, bc, dA, dB, graph=True)[0].to_host()
+    Cd = rt.spmspm_batched(ar, br, bc, dA, dB, graph=False)[0].to_host()
+    assert np.array_equal(Cg.row_ptr, Cd.row_ptr) and np.array_equal(Cg.col, Cd.col)
+
+
 def test_batched_matches_each_pair():
     ar, br, bc, A, B = rt.gen_batch_poisson(24, 16, 3000, 4.0, 32.0, 4)
     dA, dB = rt.to_device(A), rt.to_device(B)
