@@ -159,6 +159,17 @@ int rt_device_synchronize(void);
 int rt_pool_trim(void);
 int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cus);
 
+/* ---- autotuner ------------------------------------------------------------
+ * Times a set of candidate row-bin settings (bin thresholds, long-row cut,
+ * learned handling of long rows) for `mode` (RT_MODE_PLAIN or
+ * RT_MODE_LEARNED) on the given device operands, `reps` timed runs each,
+ * and records the fastest for the operands' matrix class in the tuning
+ * cache (RTSPMSPM_TUNE_CACHE or ~/.cache/rtspmspm/tune.txt).  Later calls
+ * with rt_opts.use_tuned = 1 use it.  desc receives a one-line summary.
+ * No counterpart in the reference (its pipeline is fixed at build time).   */
+int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, char* desc,
+                int64_t desc_len);
+
 /* ---- per-kernel timing (rt_opts.profile = 1) -----------------------------
  * Kernel launch durations measured with hipEvents recorded on the stream
  * each kernel is launched on, accumulated since the last reset.             */

@@ -81,6 +81,8 @@ _sig = {
     "rt_device_synchronize": ([], _c.c_int),
     "rt_pool_trim": ([], _c.c_int),
     "rt_device_info": ([_c.c_char_p, _c.c_int64, _P(_c.c_int64), _P(_c.c_int32)], _c.c_int),
+    "rt_autotune": ([_P(rt_csr), _P(rt_csr), _c.c_int32, _c.c_int32, _c.c_char_p, _c.c_int64],
+                    _c.c_int),
     "rt_prof_reset": ([], None),
     "rt_prof_count": ([], _c.c_int32),
     "rt_prof_get": ([_c.c_int32, _c.c_char_p, _c.c_int64, _P(_c.c_int64), _P(_c.c_double)], _c.c_int),
@@ -313,6 +315,15 @@ def device_copy_probe(nbytes: int) -> float:
     finally:
         _L.rt_device_free(src)
         _L.rt_device_free(dst)
+
+
+def autotune(A: DeviceCSR, B: DeviceCSR, mode: str = "plain", reps: int = 2) -> str:
+    """Time the candidate row-bin settings on these operands and cache the
+    fastest for their matrix class (used with use_tuned=True).  Returns the
+    tuner's one-line report."""
+    buf = ctypes.create_string_buffer(1024)
+    _check(_L.rt_autotune(ctypes.byref(A._s), ctypes.byref(B._s), MODES[mode], reps, buf, 1024))
+    return buf.value.decode()
 
 
 def synchronize() -> None:

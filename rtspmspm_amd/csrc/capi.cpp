@@ -160,6 +160,26 @@ struct EventPair {
   }
 };
 
+// One timed product with explicit settings (autotuner); returns device ms.
+double timed_product(const Csr& A, const Csr& B, int mode, const Tuning& tu, hipStream_t s) {
+  CsrOut C;
+  RunStats st;
+  EventPair ev;
+  RT_HIP(hipEventRecord(ev.a, s));
+  try {
+    if (mode == RT_MODE_LEARNED) spgemm_learned(A, B, Batch{}, C, tu, s, &st);
+    else spgemm_plain(A, B, Batch{}, C, tu, s, &st);
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    free_out(C);
+    throw;
+  }
+  RT_HIP(hipEventRecord(ev.b, s));
+  const double ms = ev.ms();
+  free_out(C);
+  return ms;
+}
+
 // Runs the selected method on device operands; C receives pool arrays.
 void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts& o,
                 hipStream_t s) {
@@ -172,16 +192,16 @@ void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts&
     switch (o.mode) {
       case RT_MODE_PLAIN:
         if (graph)
-          spgemm_plain_graph(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st);
+          spgemm_plain_graph(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0, o.mode), s, &st);
         else
-          spgemm_plain(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st);
+          spgemm_plain(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0, o.mode), s, &st);
         break;
       case RT_MODE_RAY:
         RT_CHECK(bt.b_base == nullptr, RT_E_INVALID, "the ray method takes a single product");
         spgemm_ray(A, B, C, o.ablation, o.ray_output, s, &st);
         break;
       case RT_MODE_LEARNED:
-        spgemm_learned(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0), s, &st);
+        spgemm_learned(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0, o.mode), s, &st);
         break;
       default: fail(RT_E_INVALID, "unknown mode " + std::to_string(o.mode));
     }
@@ -292,6 +312,48 @@ int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_row
     bt.b_base = base.p;
     run_device(Av, Bv, bt, Cd, o, s);
     export_device(Cd, C);
+  });
+}
+
+int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, char* desc,
+                int64_t desc_len) {
+  return guard([&] {
+    check_csr_header(A, "A");
+    check_csr_header(B, "B");
+    RT_CHECK(A->mem == RT_MEM_DEVICE && B->mem == RT_MEM_DEVICE, RT_E_INVALID,
+             "the autotuner takes device operands");
+    RT_CHECK(A->cols == B->rows, RT_E_SHAPE, "shape mismatch");
+    RT_CHECK(mode == RT_MODE_PLAIN || mode == RT_MODE_LEARNED, RT_E_INVALID,
+             "tune the plain or the learned method");
+    RT_CHECK(reps >= 1, RT_E_INVALID, "reps must be >= 1");
+    device();
+    hipStream_t s = lib_stream();
+    const Csr a = view(A), b = view(B);
+    validate_csr_dev(a, A->cols, nullptr, s, "A");
+    validate_csr_dev(b, B->cols, nullptr, s, "B");
+    const std::vector<Tuning> cand = tune_candidates(mode == RT_MODE_LEARNED);
+    size_t best = 0;
+    double best_ms = 0;
+    std::string report;
+    for (size_t i = 0; i < cand.size(); ++i) {
+      timed_product(a, b, mode, cand[i], s);  // warm-up (allocations, caches)
+      double t = 0;
+      for (int r = 0; r < reps; ++r) t += timed_product(a, b, mode, cand[i], s);
+      t /= reps;
+      report += (i ? ", " : "") + std::to_string(i) + ":" + std::to_string(t);
+      if (i == 0 || t < best_ms) {
+        best = i;
+        best_ms = t;
+      }
+    }
+    const std::string key = tune_key(a, b, mode);
+    record_tuning(key, cand[best]);
+    const std::string line = "class " + key + " best candidate " + std::to_string(best) + " (" +
+                             std::to_string(best_ms) + " ms); ms per candidate: " + report;
+    if (desc && desc_len > 0) {
+      std::strncpy(desc, line.c_str(), desc_len - 1);
+      desc[desc_len - 1] = '\0';
+    }
   });
 }
 

@@ -244,11 +244,12 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ p
 
 // ---- capacities, key selection, copy ---------------------------------------
 // cap = clamp(ceil(2^(pred + margin) - 1), 1, U) for every row with products;
-// rows with U <= th go to the LDS-hash learned kernels (key_light), longer
-// rows to the learned dense-tile kernel (key_heavy).
+// rows with U <= th go to the LDS-hash learned kernels (key_light); longer
+// rows go to the learned dense-tile kernel (key_heavy) when learned_heavy is
+// set, else to the exact path (key_exact = P, no capacity).
 __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__ prod, i64 ncols,
                                                   const float* __restrict__ pred, float margin,
-                                                  i64 th, i64* __restrict__ cap,
+                                                  i64 th, int learned_heavy, i64* __restrict__ cap,
                                                   i64* __restrict__ key_light,
                                                   i64* __restrict__ key_heavy,
                                                   i64* __restrict__ key_exact,
@@ -263,10 +264,15 @@ __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__
     c = (i64)ceilf(exp2f(e) - 1.f);
     c = max((i64)1, min(c, U));
   }
+  i64 ke = 0;
+  if (U > th && !learned_heavy) {  // long row on the exact path
+    ke = P;
+    c = 0;
+  }
   cap[i] = c;
   key_light[i] = U <= th ? c : 0;
   key_heavy[i] = U > th ? c : 0;
-  key_exact[i] = 0;
+  key_exact[i] = ke;
   ovf[i] = 0;
 }
 
@@ -434,8 +440,6 @@ void grad_dev(const float* params, const float* feat, const float* target, const
 
 }  // namespace
 
-Tuning active_tuning(const Csr&, const Csr&, bool) { return Tuning{}; }
-
 void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                     const Tuning& tu, hipStream_t s, RunStats* st) {
   const i64 m = A.rows;
@@ -461,7 +465,8 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   {
     ProfScope ps("capacity", s);
     hipLaunchKernelGGL(k_capacity, g256, dim3(256), 0, s, m, prod.p, B.cols, pred.p, margin,
-                       std::min(tu.sym[4], tu.num[4]), cap.p, key_light.p, key_heavy.p,
+                       std::min(tu.sym[4], tu.num[4]), tu.learned_heavy, cap.p, key_light.p,
+                       key_heavy.p,
                        key_exact.p, ovf.p);
     RT_HIP(hipGetLastError());
   }

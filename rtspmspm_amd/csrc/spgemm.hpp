@@ -36,6 +36,9 @@ struct Batch {
 struct Tuning {
   i64 sym[5] = {16, 64, 256, 1024, 4096};
   i64 num[5] = {16, 64, 256, 1024, 4096};
+  // Learned method: long rows (U > num[4]) sized by the predictor and run by
+  // the learned dense-tile kernel (1), or sent to the exact path (0).
+  int learned_heavy = 0;
 };
 
 struct RunStats {

@@ -85,3 +85,16 @@ def test_cli_ref_compat_and_errors(tmp_path):
     r = subprocess.run([CLI, "-m1", str(rect), "-m2", str(rect)], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "shape mismatch" in r.stderr
+
+
+def test_autotune_records_and_is_used(tmp_path, monkeypatch):
+    monkeypatch.setenv("RTSPMSPM_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    A = rt.gen_rmat(13, 16, seed=61)
+    dA = rt.to_device(A)
+    for mode in ("plain", "learned"):
+        report = rt.autotune(dA, dA, mode=mode, reps=1)
+        assert report.startswith("class ") and "best candidate" in report
+        C, _ = rt.spmspm(A, A, mode=mode, use_tuned=True)
+        check_parity(C, oracle.spgemm_f64(A, A))
+    lines = (tmp_path / "tune.txt").read_text().split("\n")
+    assert len([x for x in lines if x.strip()]) == 2  # one class, one line per method
