@@ -98,3 +98,28 @@ def test_autotune_records_and_is_used(tmp_path, monkeypatch):
         check_parity(C, oracle.spgemm_f64(A, A))
     lines = (tmp_path / "tune.txt").read_text().split("\n")
     assert len([x for x in lines if x.strip()]) == 2  # one class, one line per method
+
+
+def test_ae_driver_writes_harness_tables(tmp_path):
+    """The experiment driver: per-run latencies, mean/median table, speedup
+    table and JSON summary, with the ray ablation breakdown."""
+    import csv
+    import json
+    import shutil
+    data = tmp_path / "npz"
+    data.mkdir()
+    shutil.copy(os.path.join(REPO, "tests", "golden", "wiki-Vote.npz"), data)
+    out = tmp_path / "out"
+    r = subprocess.run(["python", "-m", "rtspmspm_amd.ae_bench", "--npz", str(data), "--runs", "3",
+                        "--methods", "ray,plain", "--ablations", "--out", str(out)],
+                       cwd=REPO, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = list(csv.DictReader(open(out / "all_result.csv")))
+    assert len(rows) == 6 and {x["Software"] for x in rows} == {"ray", "plain"}
+    assert all(float(x["Runtime(ms)"]) > 0 for x in rows)
+    avg = list(csv.DictReader(open(out / "avg_result.csv")))
+    assert len(avg) == 2
+    fin = list(csv.DictReader(open(out / "final_result.csv")))
+    assert fin[0]["DataSet"] == "wiki-Vote" and float(fin[0]["plain_speedup_vs_plain"]) == 1.0
+    s = json.load(open(out / "summary.json"))["wiki-Vote"]
+    assert set(s["ablation_ms"]) == {"full", "no_atomic", "traverse", "no_hit", "empty"}
