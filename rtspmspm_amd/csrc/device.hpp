@@ -9,6 +9,7 @@ namespace dev {
 
 using i64 = int64_t;
 using i32 = int32_t;
+using u16 = uint16_t;
 using u32 = uint32_t;
 using u64 = uint64_t;
 
@@ -60,6 +61,20 @@ __device__ __forceinline__ T seg_reduce_max(T v) {
     T o = __shfl_xor(v, d, W);
     v = o > v ? o : v;
   }
+  return v;
+}
+
+// Inclusive max-scan over the 64 lanes of a wave for values >= 0, by DPP:
+// row_shr 1/2/4/8 scan each row of 16 lanes (lanes shifted in from outside
+// the row read 0), then row_bcast 15 / 31 carry row 0 into row 1, row 2 into
+// row 3, and rows 0-1 into rows 2-3.  Every lane of the wave must be active.
+__device__ __forceinline__ int wave_max_scan(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
   return v;
 }
 

@@ -41,6 +41,19 @@ struct RowSet {
   __device__ __forceinline__ const i32* rows() const { return dcount ? list + doff[bin] : list; }
 };
 
+// Fills (kernels rather than memsets, so that graph capture can record them).
+__global__ __launch_bounds__(256) void k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
+}
+__global__ __launch_bounds__(256) void k_fill_i32(i32* __restrict__ p, i64 n, i32 v) {
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
+}
+
+// Zeroes the row-slot counter of a one-row-per-workgroup kernel.
+void zero_slot(i32* next, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, next, (i64)1, 0);
+}
+
 // ---------------------------------------------------------------------------
 // Row analysis: P_i = sum over A(i,k) of nnz(B(k,:)).  16 lanes per row.
 // totals[0] += sum P_i, totals[1] = max P_i.
@@ -175,6 +188,98 @@ struct ProdCursor {
   }
 };
 
+// Workgroup-wide product enumeration by owner marks, for the long-row
+// numeric kernels.  A chunk of up to WG entries A(i,k) -- entry l held by
+// thread l as (run length, first B index, A value) -- has products
+// 0..tot-1, entry e owning [ex[e], ex[e] + len[e]).  They are taken in
+// batches of ROWS*WG: every entry whose run starts inside the batch writes
+// e+1 at its start in a zeroed u16 mark array; wave w then takes the batch's
+// products [64*ROWS*w, 64*ROWS*(w+1)) as ROWS rows of 64 (lane l: 64r + l,
+// so each load instruction reads 64 consecutive products) and finds each
+// product's entry as the running maximum of the marks over the wave (a DPP
+// max-scan), seeded with the entry that covers the wave's first product (one
+// binary search per wave and batch).  Per product: one mark read, two LDS
+// reads of the entry's offset and value, two B loads; ROWS products per lane
+// are in flight.  The mark array is left zeroed.  Two barriers per batch,
+// two per scan.  OffT is the B index type (int32 when nnz(B) < 2^31).
+template <class OffT>
+struct ChunkLds {
+  OffT* off;   // [WG] B index of product p is off[e] + p
+  float* av;   // [WG] A value of the entry
+  i32* ex;     // [WG] first product of the entry
+  u16* mark;   // [ROWS*WG] zeroed owner marks
+  i32* scr;    // [WG/64] scan scratch
+};
+
+// f(col, prod) is called for every product: col = B column, prod = A*B value
+// (0 when VAL is false).  Returns the chunk's product count (uniform).
+template <int WG, int ROWS, bool VAL, class OffT, class F>
+__device__ __forceinline__ int chunk_products(int len, OffT start, float av, int nvalid,
+                                              const i32* __restrict__ bcol,
+                                              const float* __restrict__ bval,
+                                              const ChunkLds<OffT>& s, F&& f) {
+  static_assert(WG % WAVE == 0 && ROWS * WG <= 65535, "marks are u16 entry indices + 1");
+  constexpr int NB = ROWS * WG;
+  constexpr int PW = ROWS * WAVE;  // products per wave and batch
+  const int l = threadIdx.x;
+  const int lane = l & (WAVE - 1);
+  const int w = __builtin_amdgcn_readfirstlane(l / WAVE);
+  int tot;
+  const int inc = block_scan_incl<WG>(len, s.scr, &tot);
+  if (tot == 0) {
+    __syncthreads();  // every wave has read s.scr before the next scan writes it
+    return 0;
+  }
+  const int ex = inc - len;
+  s.ex[l] = ex;
+  s.off[l] = start - (OffT)ex;
+  if constexpr (VAL) s.av[l] = av;
+  for (int p0 = 0; p0 < tot; p0 += NB) {
+    if (len > 0 && ex >= p0 && ex < p0 + NB) s.mark[ex - p0] = (u16)(l + 1);
+    __syncthreads();
+    const int ws = p0 + w * PW;
+    if (ws < tot) {  // wave-uniform
+      int lo = 0, hi = nvalid - 1;  // last entry with ex <= ws
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s.ex[mid] <= ws) lo = mid; else hi = mid - 1;
+      }
+      int carry = lo + 1;
+      OffT idx[ROWS];
+      float a[ROWS];
+      bool ok[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        const int q = w * PW + r * WAVE + lane;
+        int m = s.mark[q];
+        s.mark[q] = 0;
+        m = max(wave_max_scan(m), carry);
+        carry = __builtin_amdgcn_readlane(m, WAVE - 1);
+        const int e = m - 1;
+        ok[r] = p0 + q < tot;
+        idx[r] = s.off[e] + (OffT)(p0 + q);
+        if constexpr (VAL) a[r] = s.av[e];
+      }
+      // Unconditional loads (lanes past the end read entry 0): a load under
+      // a branch makes the compiler wait for every earlier load before the
+      // next one, which would serialise the rows.
+      i32 c[ROWS];
+      float v[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        const OffT li = ok[r] ? idx[r] : (OffT)0;
+        c[r] = bcol[li];
+        if constexpr (VAL) v[r] = bval[li];
+      }
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        if (ok[r]) f(c[r], VAL ? a[r] * v[r] : 0.f);
+    }
+    __syncthreads();  // marks cleared before the next batch writes them
+  }
+  return tot;
+}
+
 // LDS-hash row kernel.  A group of G lanes owns one output row and a table of
 // T slots (linear probing, load factor <= 1/2 by binning).  The products of
 // the row are enumerated flat: a chunk of up to G entries A(i,k) is loaded,
@@ -297,55 +402,83 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
     group_sync<G>();
     const int nvalid = (int)min((i64)G, a1 - e0);
     int lo = 0;
-    for (int p = l; p < total; p += G) {
-      lo = advance_entry(s_ex + gb, lo, nvalid, p);
-      const i64 idx = s_bs[gb + lo] + (p - s_ex[gb + lo]);
-      const i32 c = bcol[idx];
-      u32 h = hash_col<LT>(c);
-      if constexpr (MODE == MODE_LEARN) {
-        // Capacity-limited insert: the row is abandoned (and flagged) as soon
-        // as it holds more distinct columns than its predicted capacity; the
-        // probe count is bounded by T as well, so no lane can spin.
-        if (s_ovf[g]) continue;
-        const float v = s_av[gb + lo] * bval[idx];
-        bool ok = false;
-        for (int probe = 0; probe < T; ++probe) {
-          i32 kk = keys[h];
-          if (kk == c) { ok = true; break; }
-          if (kk == EMPTY_KEY) {
-            kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-            if (kk == EMPTY_KEY) {
-              if (atomicAdd(&s_cnt[g], 1) >= cap) s_ovf[g] = 1;
-              ok = true;
-              break;
-            }
+    // HU products per lane per iteration: their B loads are issued together
+    // (unconditionally; lanes past the end read entry 0), then inserted.
+    constexpr int HU = 4;
+    for (int p0 = l; p0 < total; p0 += G * HU) {
+      i64 idx[HU];
+      float avu[HU];
+      bool okp[HU];
+#pragma unroll
+      for (int u = 0; u < HU; ++u) {
+        const int p = p0 + u * G;
+        okp[u] = p < total;
+        idx[u] = 0;
+        avu[u] = 0.f;
+        if (okp[u]) {
+          lo = advance_entry(s_ex + gb, lo, nvalid, p);
+          idx[u] = s_bs[gb + lo] + (p - s_ex[gb + lo]);
+          if constexpr (NUMERIC) avu[u] = s_av[gb + lo];
+        }
+      }
+      i32 cu[HU];
+      float bvu[HU];
+#pragma unroll
+      for (int u = 0; u < HU; ++u) {
+        cu[u] = bcol[idx[u]];
+        if constexpr (NUMERIC) bvu[u] = bval[idx[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < HU; ++u) {
+        if (!okp[u]) continue;
+        const i32 c = cu[u];
+        u32 h = hash_col<LT>(c);
+        if constexpr (MODE == MODE_LEARN) {
+          // Capacity-limited insert: the row is abandoned (and flagged) as
+          // soon as it holds more distinct columns than its predicted
+          // capacity; the probe count is bounded by T as well, so no lane
+          // can spin.
+          if (s_ovf[g]) continue;
+          const float v = avu[u] * bvu[u];
+          bool ok = false;
+          for (int probe = 0; probe < T; ++probe) {
+            i32 kk = keys[h];
             if (kk == c) { ok = true; break; }
+            if (kk == EMPTY_KEY) {
+              kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+              if (kk == EMPTY_KEY) {
+                if (atomicAdd(&s_cnt[g], 1) >= cap) s_ovf[g] = 1;
+                ok = true;
+                break;
+              }
+              if (kk == c) { ok = true; break; }
+            }
+            h = (h + 1) & (T - 1);
           }
-          h = (h + 1) & (T - 1);
-        }
-        if (ok) lds_add_f32(&vals[h], v); else s_ovf[g] = 1;
-      } else if constexpr (NUMERIC) {
-        const float v = s_av[gb + lo] * bval[idx];
-        while (true) {
-          i32 kk = keys[h];
-          if (kk == c) break;
-          if (kk == EMPTY_KEY) {
-            kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-            if (kk == EMPTY_KEY || kk == c) break;
-          }
-          h = (h + 1) & (T - 1);
-        }
-        lds_add_f32(&vals[h], v);
-      } else {
-        while (true) {
-          i32 kk = keys[h];
-          if (kk == c) break;
-          if (kk == EMPTY_KEY) {
-            kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-            if (kk == EMPTY_KEY) { ++mycnt; break; }
+          if (ok) lds_add_f32(&vals[h], v); else s_ovf[g] = 1;
+        } else if constexpr (NUMERIC) {
+          const float v = avu[u] * bvu[u];
+          while (true) {
+            i32 kk = keys[h];
             if (kk == c) break;
+            if (kk == EMPTY_KEY) {
+              kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+              if (kk == EMPTY_KEY || kk == c) break;
+            }
+            h = (h + 1) & (T - 1);
           }
-          h = (h + 1) & (T - 1);
+          lds_add_f32(&vals[h], v);
+        } else {
+          while (true) {
+            i32 kk = keys[h];
+            if (kk == c) break;
+            if (kk == EMPTY_KEY) {
+              kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+              if (kk == EMPTY_KEY) { ++mycnt; break; }
+              if (kk == c) break;
+            }
+            h = (h + 1) & (T - 1);
+          }
         }
       }
     }
@@ -430,6 +563,17 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Dynamic row scheduling for the one-row-per-workgroup kernels: a workgroup
+// takes its next list position from a device counter (zeroed before the
+// launch), so one that drew cheap rows takes more of them.  The grid is one
+// resident workgroup per CU.  Contains two barriers.
+__device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
+  __syncthreads();  // every thread has read the previous slot
+  if (threadIdx.x == 0) *s_slot = atomicAdd(next, 1);
+  __syncthreads();
+  return *s_slot;
+}
+
 // LDS-bitmap symbolic kernel for long rows: one workgroup per row, one bit
 // per output column over a window of 2^20 columns (128 KB of LDS); wider
 // matrices take several windows.  Also records the nnz of every 32768-column
@@ -439,7 +583,8 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
     RowSet rs, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const i64* __restrict__ brp,
     const i32* __restrict__ bcol, const i64* __restrict__ bbase, i64 ncols, i32 nt,
-    i32* __restrict__ tilecnt, i64* __restrict__ out_nnz, u32* __restrict__ rowbits) {
+    i32* __restrict__ tilecnt, i64* __restrict__ out_nnz, u32* __restrict__ rowbits,
+    i32* __restrict__ next) {
   constexpr int WG = 1024;
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
@@ -450,7 +595,8 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
   __shared__ i64 s_scr64[WG / WAVE];
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1), w = l / WAVE;
-  for (i64 it = blockIdx.x; it < nlist; it += gridDim.x) {
+  __shared__ i32 s_slot;
+  for (i64 it = next_slot(next, &s_slot); it < nlist; it = next_slot(next, &s_slot)) {
     const i32 row = list[it];
     const i64 a0 = arp[row], a1 = arp[row + 1];
     const i64 bb = BATCH ? bbase[row] : 0;
@@ -489,8 +635,7 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
           if (!any) break;
           i32 cc[UNR];
 #pragma unroll
-          for (int u = 0; u < UNR; ++u)
-            if (idx[u] >= 0) cc[u] = bcol[idx[u]];
+          for (int u = 0; u < UNR; ++u) cc[u] = bcol[idx[u] >= 0 ? idx[u] : 0];  // unconditional
 #pragma unroll
           for (int u = 0; u < UNR; ++u)
             if (idx[u] >= 0) {
@@ -562,56 +707,18 @@ constexpr int HCH = 4;
 constexpr int TC_LDS = 256;  // tile counts staged in LDS (ncols <= 8M)
 
 // Accumulates the products of one chunk of A entries (len/start/a given per
-// thread) into the LDS tile; returns the chunk's product count (uniform).
-// Each lane gathers UNR products before using them, so several B loads are
-// in flight per lane.  Two block-wide barriers.
+// thread) into the LDS tile and its presence bits; returns the chunk's
+// product count (uniform).
 template <int WG>
 __device__ __forceinline__ int tile_chunk(int len, i64 start, float av, i64 cbase,
                                           const i32* __restrict__ bcol,
                                           const float* __restrict__ bval, float* s_vals,
-                                          u32* s_bits, i64* s_bs, i32* s_ex, float* s_av,
-                                          i32* s_scr, int nvalid) {
-  const int l = threadIdx.x;
-  int tot;
-  const int inc = block_scan_incl<WG>(len, s_scr, &tot);
-  if (tot == 0) {  // uniform: nothing of this chunk falls in the tile
-    __syncthreads();  // every wave has read s_scr before the next scan writes it
-    return 0;
-  }
-  s_ex[l] = inc - len;
-  s_bs[l] = start;
-  s_av[l] = av;
-  __syncthreads();
-  ProdCursor cur;
-  cur.init<WG>(tot, nvalid, s_ex, s_bs, s_av);
-  while (true) {
-    i64 idx[UNR];
-    float a[UNR];
-    bool any = false;
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      if (cur.next(nvalid, s_ex, s_bs, s_av, idx[u], a[u])) any = true;
-      else idx[u] = -1;
-    }
-    if (!any) break;
-    i32 c[UNR];
-    float v[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u)
-      if (idx[u] >= 0) {
-        c[u] = bcol[idx[u]];
-        v[u] = bval[idx[u]];
-      }
-#pragma unroll
-    for (int u = 0; u < UNR; ++u)
-      if (idx[u] >= 0) {
-        const int cl = (int)(c[u] - cbase);
-        lds_add_f32(&s_vals[cl], a[u] * v[u]);
-        lds_or_u32(&s_bits[cl >> 5], 1u << (cl & 31));
-      }
-  }
-  __syncthreads();
-  return tot;
+                                          u32* s_bits, const ChunkLds<i64>& cl, int nvalid) {
+  return chunk_products<WG, 4, true>(len, start, av, nvalid, bcol, bval, cl, [&](i32 c, float p) {
+    const int x = (int)(c - cbase);
+    lds_add_f32(&s_vals[x], p);
+    lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
+  });
 }
 
 // Position of the r-th (0-based) set bit of b; requires r < popcount(b).
@@ -690,23 +797,28 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
     const float* __restrict__ bval, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
     float* __restrict__ cval, const i64* __restrict__ cap, i64* __restrict__ nnz,
-    i32* __restrict__ ovf) {
+    i32* __restrict__ ovf, i32* __restrict__ next) {
   constexpr int WG = 1024;
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   __shared__ float s_vals[TILE_W];
   __shared__ u32 s_bits[TILE_W / 32];
-  __shared__ i32 s_pfx[TILE_W / 32];
-  __shared__ i64 s_bs[WG];
+  __shared__ i64 s_off[WG];
   __shared__ i32 s_ex[WG];
   __shared__ float s_av[WG];
+  __shared__ u16 s_mark[4 * WG];
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
+  // The emit's word prefix reuses s_off (rewritten at every chunk start).
+  i32* s_pfx = reinterpret_cast<i32*>(s_off);
+  const ChunkLds<i64> cl{s_off, s_av, s_ex, s_mark, s_scr};
   const int l = threadIdx.x;
   for (int d = l; d < TILE_W; d += WG) s_vals[d] = 0.f;
+  for (int d = l; d < 4 * WG; d += WG) s_mark[d] = 0;
   s_bits[l] = 0;
   __syncthreads();
-  for (i64 h = blockIdx.x; h < nlist; h += gridDim.x) {
+  __shared__ i32 s_slot;
+  for (i64 h = next_slot(next, &s_slot); h < nlist; h = next_slot(next, &s_slot)) {
     const i32 row = list[h];
     if (key[row] <= th) continue;
     const i32* tc = LEARN ? nullptr : tilecnt + h * nt;
@@ -756,7 +868,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
             if (c < nch) {
               const int nvalid = (int)min((i64)WG, nA - (i64)c * WG);
               any += tile_chunk<WG>(ok[c] ? nxt[c] - cur[c] : 0, b0[c] + cur[c], av[c], cbase,
-                                    bcol, bval, s_vals, s_bits, s_bs, s_ex, s_av, s_scr, nvalid);
+                                    bcol, bval, s_vals, s_bits, cl, nvalid);
             }
           }
           if (any > 0) {
@@ -791,8 +903,8 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
             len = s1 - s0;
             a = aval[e];
           }
-          any += tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_bits, s_bs, s_ex,
-                                s_av, s_scr, (int)min((i64)WG, a1 - e0));
+          any += tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_bits, cl,
+                                (int)min((i64)WG, a1 - e0));
         }
         if (any > 0) {
           const int n = tile_emit<WG>(out, cbase, capacity - (out - base), s_vals, s_bits, s_pfx,
@@ -819,71 +931,72 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
 // tiles holding at most SEG_CAP entries is one "compressed" step -- its
 // bitmap words and their popcount prefix are staged in LDS, every product is
 // added at its rank (prefix + popcount of the lower bits of its word), so the
-// accumulators already sit in column order and are written out directly.  A
-// tile holding more than SEG_CAP entries is one "dense" step (the tile
-// accumulator of k_num_heavy).  One workgroup per row, rows grid-stride.
+// accumulators already sit in column order.  A tile holding more than
+// SEG_CAP entries is one "dense" step: 32768 LDS accumulators indexed by
+// column.  Either way thread l owns bitmap words of the step (eight of a
+// compressed step, one of a dense tile) and writes their entries to C itself,
+// at the word's prefix: the output needs no search and no sort.  One
+// workgroup per row, rows grid-stride.
+//
+// Rows of up to HC*1024 entries keep their entries in registers for the
+// whole row: B row start, slab row, A value, and a cursor into the B row
+// that advances from step to step (a step's B range of entry k ends at
+// slab[k][t1], and the next step starts there, because the tiles skipped in
+// between hold no product of the row).  The next step's range ends and
+// bitmap words are loaded while the current step runs.  Longer rows re-read
+// their entries per step and chunk.
 constexpr int SEG_TILES = 8;
 constexpr int SEG_WORDS = SEG_TILES * (TILE_W / 32);  // 8192 bitmap words
 constexpr int SEG_CAP = 16384;                        // entries per compressed step
-// LDS map (bytes): [0, 128K) dense accumulators; in compressed steps
-// [0, 64K) accumulators and [64K, 128K) one u64 per bitmap word: the
-// word's popcount prefix in the high half, its bits in the low half.
-// [128K, 132K) dense presence bits, [132K, 136K) dense word prefix.
-constexpr int HSEG_LDS = 136 * 1024;
+// LDS map of the accumulator region (bytes): dense steps use [0, 128K) as
+// 32768 accumulators; compressed steps use [0, 64K) as accumulators and
+// [64K, 128K) as one u64 per bitmap word: the word's popcount prefix in the
+// high half, its bits in the low half.
+constexpr int HSEG_LDS = 128 * 1024;
 
-// Products of one chunk restricted to the segment's column range, added at
-// their rank.  Two block-wide barriers; returns the product count.
-template <int WG>
-__device__ __forceinline__ int seg_chunk(int len, i64 start, float av, i64 cbase,
-                                         const i32* __restrict__ bcol,
-                                         const float* __restrict__ bval, float* s_vals,
-                                         const u64* s_cw, i64* s_bs,
-                                         i32* s_ex, float* s_av, i32* s_scr, int nvalid) {
-  const int l = threadIdx.x;
-  int tot;
-  const int inc = block_scan_incl<WG>(len, s_scr, &tot);
-  if (tot == 0) {
-    __syncthreads();
-    return 0;
-  }
-  s_ex[l] = inc - len;
-  s_bs[l] = start;
-  s_av[l] = av;
-  __syncthreads();
-  ProdCursor cur;
-  cur.init<WG>(tot, nvalid, s_ex, s_bs, s_av);
-  while (true) {
-    i64 idx[UNR];
-    float a[UNR];
-    bool any = false;
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      if (cur.next(nvalid, s_ex, s_bs, s_av, idx[u], a[u])) any = true;
-      else idx[u] = -1;
+// Step [t, t1) of a row from tile t (whose count c0 > 0): n entries, dense
+// when the tile alone exceeds SEG_CAP.
+__device__ __forceinline__ void seg_step(const i32* s_tc, const i32* tc, int nt, int t, int& t1,
+                                         int& n, bool& dense) {
+  const int c0 = t < TC_LDS ? s_tc[t] : tc[t];
+  t1 = t + 1;
+  n = c0;
+  dense = c0 > SEG_CAP;
+  if (!dense) {
+    while (t1 < nt && t1 - t < SEG_TILES) {
+      const int ct = t1 < TC_LDS ? s_tc[t1] : tc[t1];
+      if (ct > SEG_CAP || n + ct > SEG_CAP) break;
+      n += ct;
+      ++t1;
     }
-    if (!any) break;
-    i32 c[UNR];
-    float v[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u)
-      if (idx[u] >= 0) {
-        c[u] = bcol[idx[u]];
-        v[u] = bval[idx[u]];
-      }
-#pragma unroll
-    for (int u = 0; u < UNR; ++u)
-      if (idx[u] >= 0) {
-        const int cl = (int)(c[u] - cbase);
-        const u64 wd = s_cw[cl >> 5];  // prefix in the high half, bits in the low half
-        const int rank = (int)(wd >> 32) + __popc((u32)wd & ((1u << (cl & 31)) - 1u));
-        lds_add_f32(&s_vals[rank], a[u] * v[u]);
-      }
   }
-  __syncthreads();
-  return tot;
 }
 
-template <bool BATCH>
+__device__ __forceinline__ int next_nonempty(const i32* s_tc, const i32* tc, int nt, int t) {
+  while (t < nt && (t < TC_LDS ? s_tc[t] : tc[t]) == 0) ++t;
+  return t;
+}
+
+// This thread's bitmap words of step [t, t1): words 8l..8l+7 of a compressed
+// step, word l of a dense tile (in w[0]).
+__device__ __forceinline__ void seg_words(const u32* bits, int t, int t1, bool dense, u32 (&w)[8]) {
+  const int l = threadIdx.x;
+  const u32* src = bits + (i64)t * (TILE_W / 32);
+  if (dense) {
+    w[0] = src[l];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) w[j] = 0u;
+  } else {
+    const int nw = (t1 - t) * (TILE_W / 32);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int wi = l * 8 + j;
+      w[j] = wi < nw ? src[wi] : 0u;
+    }
+  }
+}
+
+template <bool BATCH, class OffT, int HC, int ROWS>
 __global__ __launch_bounds__(1024) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
@@ -891,123 +1004,213 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
     const i64* __restrict__ brp, const i32* __restrict__ bcol,
     const float* __restrict__ bval, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
-    float* __restrict__ cval) {
+    float* __restrict__ cval, i32* __restrict__ next) {
   constexpr int WG = 1024;
-  constexpr int WPT = SEG_WORDS / WG;  // bitmap words staged per thread
-  static_assert(TILE_W / 32 == WG && WPT == 8, "layout assumes 1024 threads");
+  static_assert(TILE_W / 32 == WG && SEG_WORDS / WG == 8, "layout assumes 1024 threads");
   __shared__ __align__(16) unsigned char s_raw[HSEG_LDS];
-  __shared__ i64 s_bs[WG];
-  __shared__ i32 s_ex[WG];
+  __shared__ OffT s_off[WG];
   __shared__ float s_av[WG];
+  __shared__ i32 s_ex[WG];
+  __shared__ u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
-  float* s_vals = reinterpret_cast<float*>(s_raw);
+  float* s_acc = reinterpret_cast<float*>(s_raw);
   u64* s_cw = reinterpret_cast<u64*>(s_raw + 64 * 1024);
-  u32* s_dbits = reinterpret_cast<u32*>(s_raw + 128 * 1024);
-  i32* s_dpfx = reinterpret_cast<i32*>(s_raw + 132 * 1024);
+  const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_scr};
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   const int l = threadIdx.x;
   const i64 row_words = (i64)nt * (TILE_W / 32);
-  for (int d = l; d < TILE_W; d += WG) s_vals[d] = 0.f;
-  s_dbits[l] = 0;
+  for (int d = l; d < TILE_W; d += WG) s_acc[d] = 0.f;
+  for (int d = l; d < ROWS * WG; d += WG) s_mark[d] = 0;
+  bool upper_dirty = false;  // s_acc[SEG_CAP, TILE_W) holds staged words
   __syncthreads();
-  for (i64 h = blockIdx.x; h < nlist; h += gridDim.x) {
+  __shared__ i32 s_slot;
+  for (i64 h = next_slot(next, &s_slot); h < nlist; h = next_slot(next, &s_slot)) {
     const i32 row = list[h];
     if (key[row] <= th) continue;
     const i32* tc = tilecnt + h * nt;
     for (int t = l; t < min(nt, TC_LDS); t += WG) s_tc[t] = tc[t];
-    const i64 a0 = arp[row], a1 = arp[row + 1];
+    const i64 a0 = arp[row], a1 = arp[row + 1], nA = a1 - a0;
     const i64 bb = BATCH ? bbase[row] : 0;
     const u32* bits = rowbits + h * row_words;
     i64 out = crp[row];
-    __syncthreads();
-    int t = 0;
-    while (t < nt) {
-      const int c0 = t < TC_LDS ? s_tc[t] : tc[t];
-      if (c0 == 0) {
-        ++t;
-        continue;
+    const bool inreg = nA <= (i64)HC * WG;
+    const int nch = (int)((nA + WG - 1) / WG);
+    bool ok[HC];
+    float av[HC];
+    OffT b0[HC];
+    const i32* sp[HC];
+    i32 cur[HC], nxt[HC];
+    if (inreg) {
+#pragma unroll
+      for (int c = 0; c < HC; ++c) {
+        const i64 e = a0 + (i64)c * WG + l;
+        ok[c] = e < a1;
+        av[c] = 0.f;
+        b0[c] = 0;
+        sp[c] = slab;
+        cur[c] = nxt[c] = 0;
+        if (ok[c]) {
+          const i64 k = (i64)acol[e] + bb;
+          av[c] = aval[e];
+          b0[c] = (OffT)brp[k];
+          sp[c] = slab + k * (nt + 1);
+        }
       }
-      int t1 = t + 1, n = c0;
-      const bool dense = c0 > SEG_CAP;
-      if (!dense) {
-        while (t1 < nt && t1 - t < SEG_TILES) {
-          const int ct = t1 < TC_LDS ? s_tc[t1] : tc[t1];
-          if (ct > SEG_CAP || n + ct > SEG_CAP) break;
-          n += ct;
-          ++t1;
-        }
-        // Stage the segment's bitmap words and their prefix.
-        const int nw = (t1 - t) * (TILE_W / 32);
-        const u32* src = bits + (i64)t * (TILE_W / 32);
-        u32 wv[WPT];
-        int cnt = 0;
+    }
+    __syncthreads();  // s_tc
+    int t = next_nonempty(s_tc, tc, nt, 0);
+    int t1 = t, n = 0;
+    bool dense = false;
+    u32 wv[8];
+    if (t < nt) {
+      seg_step(s_tc, tc, nt, t, t1, n, dense);
+      seg_words(bits, t, t1, dense, wv);
+      if (inreg) {
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) {
-          const int w = l * WPT + j;
-          wv[j] = w < nw ? src[w] : 0u;
-          cnt += __popc(wv[j]);
-        }
-        int tot;
-        const int inc = block_scan_incl<WG>(cnt, s_scr, &tot);
-        int run = inc - cnt;
+        for (int c = 0; c < HC; ++c) nxt[c] = ok[c] ? sp[c][t1] : 0;
+      }
+    }
+    while (t < nt) {
+      // The next step: its extent, range ends and bitmap words (in flight
+      // while this step runs).
+      const int tn = next_nonempty(s_tc, tc, nt, t1);
+      int tn1 = tn, nn = 0;
+      bool dn = false;
+      i32 pre[HC];
+      u32 nwv[8];
 #pragma unroll
-        for (int j = 0; j < WPT; ++j) {
-          s_cw[l * WPT + j] = ((u64)(u32)run << 32) | wv[j];
-          run += __popc(wv[j]);
+      for (int c = 0; c < HC; ++c) pre[c] = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) nwv[j] = 0u;
+      if (tn < nt) {
+        seg_step(s_tc, tc, nt, tn, tn1, nn, dn);
+        seg_words(bits, tn, tn1, dn, nwv);
+        if (inreg) {
+#pragma unroll
+          for (int c = 0; c < HC; ++c) pre[c] = ok[c] ? sp[c][tn1] : 0;
         }
-        __syncthreads();
       }
       const i64 cbase = (i64)t * TILE_W;
-      // Products of every A entry inside tiles [t, t1).
-      for (i64 e0 = a0; e0 < a1; e0 += WG) {
-        const i64 e = e0 + l;
-        int len = 0;
-        i64 start = 0;
-        float a = 0.f;
-        if (e < a1) {
-          const i64 k = (i64)acol[e] + bb;
-          const i32* sk = slab + k * (nt + 1);
-          const i32 s0 = sk[t], s1 = sk[t1];
-          start = brp[k] + s0;
-          len = s1 - s0;
-          a = aval[e];
-        }
-        const int nvalid = (int)min((i64)WG, a1 - e0);
-        if (dense)
-          tile_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_dbits, s_bs, s_ex, s_av,
-                         s_scr, nvalid);
-        else
-          seg_chunk<WG>(len, start, a, cbase, bcol, bval, s_vals, s_cw, s_bs, s_ex,
-                        s_av, s_scr, nvalid);
-      }
+      int wpfx = 0;  // compressed: prefix of this thread's first word
       if (dense) {
-        tile_emit<WG>(out, cbase, LLONG_MAX, s_vals, s_dbits, s_dpfx, s_scr, ccol, cval);
-      } else {
-        // Accumulators are in column order: entry q is the q-th set bit.
-        const int nw = (t1 - t) * (TILE_W / 32);
-        int w = 0;
-        for (int q = l; q < n; q += WG) {
-          int hi = nw - 1;
-          while (w < hi) {
-            const int mid = (w + hi + 1) >> 1;
-            if ((int)(s_cw[mid] >> 32) <= q) w = mid; else hi = mid - 1;
-          }
-          const u64 wd = s_cw[w];
-          const int bit = select_bit((u32)wd, q - (int)(wd >> 32));
-          ccol[out + q] = (i32)(cbase + w * 32 + bit);
-          cval[out + q] = s_vals[q];
-          s_vals[q] = 0.f;
+        if (upper_dirty) {
+          __syncthreads();  // the previous step's output has read its words
+          for (int d = SEG_CAP + l; d < TILE_W; d += WG) s_acc[d] = 0.f;
+          upper_dirty = false;
+          __syncthreads();
         }
+      } else {
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cnt += __popc(wv[j]);
+        int tot;
+        const int inc = block_scan_incl<WG>(cnt, s_scr, &tot);
+        wpfx = inc - cnt;
+        int run = wpfx;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          s_cw[l * 8 + j] = ((u64)(u32)run << 32) | wv[j];
+          run += __popc(wv[j]);
+        }
+        upper_dirty = true;
         __syncthreads();
-        // The next step may be dense: its accumulators span the region the
-        // bitmap and prefix used.
-        for (int d = SEG_CAP + l; d < TILE_W; d += WG) s_vals[d] = 0.f;
-        __syncthreads();
+      }
+      auto add_dense = [&](i32 c, float p) { lds_add_f32(&s_acc[(int)(c - cbase)], p); };
+      auto add_rank = [&](i32 c, float p) {
+        const int x = (int)(c - cbase);
+        const u64 wd = s_cw[x >> 5];  // prefix in the high half, bits in the low half
+        const int rank = (int)(wd >> 32) + __popc((u32)wd & ((1u << (x & 31)) - 1u));
+        lds_add_f32(&s_acc[rank], p);
+      };
+      if (inreg) {
+        for (int c = 0; c < nch; ++c) {
+          // c is uniform; the unrolled register arrays are selected by it
+          int len = 0;
+          OffT start = 0;
+          float a = 0.f;
+#pragma unroll
+          for (int cc = 0; cc < HC; ++cc)
+            if (cc == c && ok[cc]) {
+              len = nxt[cc] - cur[cc];
+              start = b0[cc] + (OffT)cur[cc];
+              a = av[cc];
+            }
+          const int nvalid = (int)min((i64)WG, nA - (i64)c * WG);
+          if (dense)
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_dense);
+          else
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_rank);
+        }
+#pragma unroll
+        for (int c = 0; c < HC; ++c) {
+          cur[c] = nxt[c];
+          nxt[c] = pre[c];
+        }
+      } else {
+        for (i64 e0 = a0; e0 < a1; e0 += WG) {
+          const i64 e = e0 + l;
+          int len = 0;
+          OffT start = 0;
+          float a = 0.f;
+          if (e < a1) {
+            const i64 k = (i64)acol[e] + bb;
+            const i32* sk = slab + k * (nt + 1);
+            const i32 s0 = sk[t], s1 = sk[t1];
+            start = (OffT)(brp[k] + s0);
+            len = s1 - s0;
+            a = aval[e];
+          }
+          const int nvalid = (int)min((i64)WG, a1 - e0);
+          if (dense)
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_dense);
+          else
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_rank);
+        }
+      }
+      // Output: every thread writes the entries of its own words.
+      if (dense) {
+        const u32 b = wv[0];
+        const int pcb = (int)__popc(b);
+        int tot;
+        const int inc = block_scan_incl<WG>(pcb, s_scr, &tot);
+        i64 q = out + (inc - pcb);
+        u32 r = b;
+        while (r) {
+          const int x = l * 32 + (__ffs(r) - 1);
+          r &= r - 1;
+          ccol[q] = (i32)(cbase + x);
+          cval[q] = s_acc[x];
+          s_acc[x] = 0.f;
+          ++q;
+        }
+        __syncthreads();  // s_scr is reused by the next scan
+      } else {
+        int q = wpfx;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          u32 r = wv[j];
+          const i64 cw = cbase + (i64)(l * 8 + j) * 32;
+          while (r) {
+            const int bit = __ffs(r) - 1;
+            r &= r - 1;
+            ccol[out + q] = (i32)(cw + bit);
+            cval[out + q] = s_acc[q];
+            s_acc[q] = 0.f;
+            ++q;
+          }
+        }
+        // No barrier: the next step rewrites only this thread's words, and
+        // its accumulation starts after the barriers of its staging scan.
       }
       out += n;
-      t = t1;
+      t = tn;
+      t1 = tn1;
+      n = nn;
+      dense = dn;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) wv[j] = nwv[j];
     }
     __syncthreads();  // s_tc is reloaded for the next row
   }
@@ -1227,11 +1430,14 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     if (bm_bytes <= free_b / 3 + pool_cached_bytes() / 3)
       heavy.rowbits.alloc(bm_bytes / sizeof(u32));
-    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 8);
+    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus);
+    heavy.next.alloc(1);
+    zero_slot(heavy.next.p, s);
     ProfScope ps("sym_bitmap", s);
     const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
     hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, rs, A.rp, A.col, B.rp,
-                       B.col, bbase, B.cols, heavy.nt, heavy.tilecnt.p, nnz, heavy.rowbits.p);
+                       B.col, bbase, B.cols, heavy.nt, heavy.tilecnt.p, nnz, heavy.rowbits.p,
+                       heavy.next.p);
     RT_HIP(hipGetLastError());
   }
 }
@@ -1253,19 +1459,27 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
                          B.rows, heavy.nt, B.rp, B.col, slab.p);
       RT_HIP(hipGetLastError());
     }
-    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 4);
+    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus);
+    DBuf<i32> next(1);
+    zero_slot(next.p, s);
     {
       ProfScope ps("num_heavy", s);
       const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
-      if (heavy.rowbits.p)
-        hipLaunchKernelGGL(k_num_heavy_seg<BATCH>, dim3(grid), dim3(1024), 0, s, rs, heavy.nt,
-                           heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col, A.val,
-                           B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val);
+      // int32 B offsets (and eight products per lane in flight) when B's
+      // entries are indexable by int32.
+      if (heavy.rowbits.p && B.nnz < ((i64)1 << 31))
+        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 2, 8>), dim3(grid), dim3(1024), 0, s, rs,
+                           heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
+                           A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, next.p);
+      else if (heavy.rowbits.p)
+        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 4, 4>), dim3(grid), dim3(1024), 0, s, rs,
+                           heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
+                           A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, next.p);
       else
         hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(1024), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val, B.rp,
                            B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr,
-                           nullptr);
+                           nullptr, next.p);
       RT_HIP(hipGetLastError());
     }
     HashArgs a = make_args(A, B, bbase);
@@ -1393,19 +1607,21 @@ void learned_heavy_pass(const Csr& A, const Csr& B, const i64* bbase, const i64*
                        B.rows, nt, B.rp, B.col, slab.p);
     RT_HIP(hipGetLastError());
   }
-  const unsigned grid = (unsigned)std::min<i64>(cnt[5], (i64)device().cus * 4);
+  const unsigned grid = (unsigned)std::min<i64>(cnt[5], (i64)device().cus);
+  DBuf<i32> next(1);
+  zero_slot(next.p, s);
   {
     ProfScope ps("learn_heavy", s);
     if (bbase)
       hipLaunchKernelGGL((k_num_heavy<true, true>), dim3(grid), dim3(1024), 0, s,
                          RowSet{list.p + off[5], cnt[5], nullptr, nullptr, 0}, nt, nullptr, key,
                          (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p, tmp_off,
-                         tcol, tval, cap, nnz, ovf);
+                         tcol, tval, cap, nnz, ovf, next.p);
     else
       hipLaunchKernelGGL((k_num_heavy<false, true>), dim3(grid), dim3(1024), 0, s,
                          RowSet{list.p + off[5], cnt[5], nullptr, nullptr, 0}, nt, nullptr, key,
                          (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, nullptr, slab.p, tmp_off,
-                         tcol, tval, cap, nnz, ovf);
+                         tcol, tval, cap, nnz, ovf, next.p);
     RT_HIP(hipGetLastError());
   }
   RT_HIP(hipStreamSynchronize(s));  // list and slab are released at scope exit
@@ -1449,12 +1665,6 @@ void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
 namespace rt {
 using namespace dev;
 
-__global__ __launch_bounds__(256) void k_fill_i64(i64* __restrict__ p, i64 n, i64 v) {
-  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
-}
-__global__ __launch_bounds__(256) void k_fill_i32(i32* __restrict__ p, i64 n, i32 v) {
-  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
-}
 // off[b] = exclusive prefix of cnt, cursor[b] = off[b].
 __global__ void k_bin_offsets(const i32* __restrict__ cnt, i32* __restrict__ off,
                               i32* __restrict__ cursor) {
@@ -1573,7 +1783,8 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
   C.cols = B.cols;
   C.rp = static_cast<i64*>(dmalloc((m + 1) * sizeof(i64)));
   HashArgs base = make_args(A, B, bbase);
-  const unsigned heavy_grid = (unsigned)device().cus * 4;
+  const unsigned heavy_grid = (unsigned)device().cus;
+  DBuf<i32> next(2);  // row-slot counters of the two long-row kernels
   double ms_a = 0, ms_b = 0;
   try {
     ms_a = run_captured(s, [&] {
@@ -1588,8 +1799,9 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       a.nnz = nnz.p;
       launch_hash_bins_dev<MODE_SYM, BATCH>(a, sb, m, s);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
-      hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(heavy_grid * 2), dim3(1024), 0, s, heavy, A.rp,
-                         A.col, B.rp, B.col, bbase, B.cols, nt, tilecnt.p, nnz.p, nullptr);
+      zero_slot(next.p, s);
+      hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(heavy_grid), dim3(1024), 0, s, heavy, A.rp,
+                         A.col, B.rp, B.col, bbase, B.cols, nt, tilecnt.p, nnz.p, nullptr, next.p);
       hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
       hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nbs, C.rp + m);
       if (m > 0)
@@ -1607,9 +1819,10 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
                            dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)), dim3(256), 0,
                            s, B.rows, nt, B.rp, B.col, slab.p);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
+      zero_slot(next.p + 1, s);
       hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(heavy_grid), dim3(1024), 0, s, heavy,
                          nt, tilecnt.p, nnz.p, tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val,
-                         bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr, nullptr);
+                         bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr, nullptr, next.p + 1);
       HashArgs a = base;
       a.crp = C.rp;
       a.ccol = C.col;

@@ -55,6 +55,7 @@ struct HeavyRows {
   i32 off = 0, n = 0, nt = 1;
   DBuf<i32> tilecnt;  // n x nt
   DBuf<u32> rowbits;  // n x nt x 1024 words: each long row's column set (may be empty)
+  DBuf<i32> next;     // row-slot counter of the symbolic long-row kernel
 };
 
 // Pipeline pieces.  `key` arrays select and size rows: rows with key 0 are
