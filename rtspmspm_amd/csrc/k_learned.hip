@@ -243,13 +243,21 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float* __restrict__ p
 }
 
 // ---- capacities, key selection, copy ---------------------------------------
-// cap = clamp(ceil(2^(pred + margin) - 1), 1, U) for every row with products;
-// rows with U <= th go to the LDS-hash learned kernels (key_light); longer
-// rows go to the learned dense-tile kernel (key_heavy) when learned_heavy is
-// set, else to the exact path (key_exact = P, no capacity).
+// cap = clamp(ceil(2^(pred + margin) - 1), 1, U) for every row with products,
+// then raised to the upper bound of its row bin (bt[b], the table a row of
+// that bin gets holds twice that), still at most U: the slack is free, so a
+// row overflows only when it outgrows the table it is given.  Rows with
+// U <= th go to the LDS-hash learned kernels (key_light); longer rows go to
+// the learned dense-tile kernel (key_heavy) when learned_heavy is set, else
+// to the exact path (key_exact = P, no capacity).
+struct BinTop {
+  i64 t[5];
+};
+
 __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__ prod, i64 ncols,
                                                   const float* __restrict__ pred, float margin,
-                                                  i64 th, int learned_heavy, i64* __restrict__ cap,
+                                                  BinTop bt, i64 th, int learned_heavy,
+                                                  i64* __restrict__ cap,
                                                   i64* __restrict__ key_light,
                                                   i64* __restrict__ key_heavy,
                                                   i64* __restrict__ key_exact,
@@ -263,6 +271,12 @@ __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__
     const float e = fminf(pred[i] + margin, 40.f);
     c = (i64)ceilf(exp2f(e) - 1.f);
     c = max((i64)1, min(c, U));
+#pragma unroll
+    for (int b = 0; b < 5; ++b)
+      if (c <= bt.t[b]) {
+        c = min(bt.t[b], U);
+        break;
+      }
   }
   i64 ke = 0;
   if (U > th && !learned_heavy) {  // long row on the exact path
@@ -464,7 +478,9 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   const dim3 g256((unsigned)((M + 255) / 256));
   {
     ProfScope ps("capacity", s);
-    hipLaunchKernelGGL(k_capacity, g256, dim3(256), 0, s, m, prod.p, B.cols, pred.p, margin,
+    BinTop bt;
+    for (int b = 0; b < 5; ++b) bt.t[b] = tu.num[b];
+    hipLaunchKernelGGL(k_capacity, g256, dim3(256), 0, s, m, prod.p, B.cols, pred.p, margin, bt,
                        std::min(tu.sym[4], tu.num[4]), tu.learned_heavy, cap.p, key_light.p,
                        key_heavy.p,
                        key_exact.p, ovf.p);

@@ -55,8 +55,9 @@ void zero_slot(i32* next, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Row analysis: P_i = sum over A(i,k) of nnz(B(k,:)).  16 lanes per row.
-// totals[0] += sum P_i, totals[1] = max P_i.
+// Row analysis: P_i = sum over A(i,k) of nnz(B(k,:)).  16 lanes per row,
+// rows grid-stride; totals[0] += sum P_i, totals[1] = max P_i, with one
+// atomic of each kind per workgroup (same-address atomics serialise).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_row_products(
     i64 nrows, const i64* __restrict__ arp, const i32* __restrict__ acol,
@@ -64,23 +65,36 @@ __global__ __launch_bounds__(256) void k_row_products(
     unsigned long long* __restrict__ totals) {
   constexpr int G = 16;
   __shared__ i64 s_scr[4];
-  const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / G;
   const int l = threadIdx.x & (G - 1);
-  i64 sum = 0;
-  if (row < nrows) {
-    const i64 a0 = arp[row], a1 = arp[row + 1];
-    const i64 bb = BATCH ? bbase[row] : 0;
-    for (i64 e = a0 + l; e < a1; e += G) {
-      const i64 k = (i64)acol[e] + bb;
-      sum += brp[k + 1] - brp[k];
+  i64 tsum = 0, tmax = 0;
+  for (i64 r0 = (i64)blockIdx.x * (256 / G); r0 < nrows; r0 += (i64)gridDim.x * (256 / G)) {
+    const i64 row = r0 + threadIdx.x / G;
+    i64 sum = 0;
+    if (row < nrows) {
+      const i64 a0 = arp[row], a1 = arp[row + 1];
+      const i64 bb = BATCH ? bbase[row] : 0;
+      for (i64 e = a0 + l; e < a1; e += G) {
+        const i64 k = (i64)acol[e] + bb;
+        sum += brp[k + 1] - brp[k];
+      }
+    }
+    sum = seg_reduce_sum<G>(sum);
+    if (row < nrows && l == 0) prod[row] = sum;
+    if (l == 0) {
+      tsum += sum;
+      tmax = max(tmax, sum);
     }
   }
-  sum = seg_reduce_sum<G>(sum);
-  if (row < nrows && l == 0) prod[row] = sum;
-  i64 mx = seg_reduce_max<WAVE>(sum);
-  i64 ws = block_reduce_sum<256>(l == 0 ? sum : (i64)0, s_scr);
-  if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(&totals[1], (unsigned long long)mx);
-  if (threadIdx.x == 0) atomicAdd(&totals[0], (unsigned long long)ws);
+  const i64 mx = seg_reduce_max<WAVE>(tmax);
+  const i64 ws = block_reduce_sum<256>(tsum, s_scr);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && mx > 0) atomicMax(&totals[1], (unsigned long long)mx);
+  if (threadIdx.x == 0 && ws > 0) atomicAdd(&totals[0], (unsigned long long)ws);
+}
+
+// Grid of the row-analysis kernel: enough workgroups to fill the device.
+inline unsigned row_products_grid(i64 nrows) {
+  const i64 need = (nrows * 16 + 255) / 256;
+  return (unsigned)std::max<i64>(1, std::min<i64>(need, (i64)device().cus * 16));
 }
 
 // ---------------------------------------------------------------------------
@@ -188,26 +202,27 @@ struct ProdCursor {
   }
 };
 
-// Workgroup-wide product enumeration by owner marks, for the long-row
-// numeric kernels.  A chunk of up to WG entries A(i,k) -- entry l held by
-// thread l as (run length, first B index, A value) -- has products
-// 0..tot-1, entry e owning [ex[e], ex[e] + len[e]).  They are taken in
-// batches of ROWS*WG: every entry whose run starts inside the batch writes
-// e+1 at its start in a zeroed u16 mark array; wave w then takes the batch's
-// products [64*ROWS*w, 64*ROWS*(w+1)) as ROWS rows of 64 (lane l: 64r + l,
-// so each load instruction reads 64 consecutive products) and finds each
-// product's entry as the running maximum of the marks over the wave (a DPP
-// max-scan), seeded with the entry that covers the wave's first product (one
-// binary search per wave and batch).  Per product: one mark read, two LDS
-// reads of the entry's offset and value, two B loads; ROWS products per lane
-// are in flight.  The mark array is left zeroed.  Two barriers per batch,
-// two per scan.  OffT is the B index type (int32 when nnz(B) < 2^31).
+// Product enumeration by owner marks, for the long-row numeric kernels.  A
+// chunk of up to WG entries A(i,k) -- entry l held by thread l as (run
+// length, first B index, A value) -- has products 0..tot-1, entry e owning
+// [ex[e], ex[e] + len[e]).  After one block scan the waves work
+// independently: wave w takes an equal contiguous slice of the products and
+// walks it in sub-batches of 64*ROWS (lane l: products 64r + l of the
+// sub-batch, so each load instruction reads 64 consecutive products).  For a
+// sub-batch the wave marks, in its own zeroed u16 array, the start of every
+// nonempty entry that begins inside it (e+1 at ex[e]); each product's entry
+// is then the running maximum of the marks (a DPP max-scan over the wave),
+// seeded with the entry that covers the sub-batch's first product.  Per
+// product: one mark read, two LDS reads of the entry's offset and value, two
+// B loads; ROWS products per lane are in flight, and no wave waits for
+// another inside the chunk.  OffT is the B index type (int32 when nnz(B) <
+// 2^31).  Four barriers per chunk (two in the scan).
 template <class OffT>
 struct ChunkLds {
   OffT* off;   // [WG] B index of product p is off[e] + p
   float* av;   // [WG] A value of the entry
-  i32* ex;     // [WG] first product of the entry
-  u16* mark;   // [ROWS*WG] zeroed owner marks
+  i32* ex;     // [WG+1] first product of the entry; ex[nvalid] = tot
+  u16* mark;   // [WG/64][64*ROWS] per-wave zeroed owner marks
   i32* scr;    // [WG/64] scan scratch
 };
 
@@ -218,9 +233,9 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
                                               const i32* __restrict__ bcol,
                                               const float* __restrict__ bval,
                                               const ChunkLds<OffT>& s, F&& f) {
-  static_assert(WG % WAVE == 0 && ROWS * WG <= 65535, "marks are u16 entry indices + 1");
-  constexpr int NB = ROWS * WG;
-  constexpr int PW = ROWS * WAVE;  // products per wave and batch
+  static_assert(WG % WAVE == 0 && WG <= 65535, "marks are u16 entry indices + 1");
+  constexpr int NW = WG / WAVE;
+  constexpr int SB = ROWS * WAVE;  // products per sub-batch
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1);
   const int w = __builtin_amdgcn_readfirstlane(l / WAVE);
@@ -232,34 +247,46 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
   }
   const int ex = inc - len;
   s.ex[l] = ex;
+  if (l == 0) s.ex[nvalid] = tot;
   s.off[l] = start - (OffT)ex;
   if constexpr (VAL) s.av[l] = av;
-  for (int p0 = 0; p0 < tot; p0 += NB) {
-    if (len > 0 && ex >= p0 && ex < p0 + NB) s.mark[ex - p0] = (u16)(l + 1);
-    __syncthreads();
-    const int ws = p0 + w * PW;
-    if (ws < tot) {  // wave-uniform
-      int lo = 0, hi = nvalid - 1;  // last entry with ex <= ws
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s.ex[mid] <= ws) lo = mid; else hi = mid - 1;
+  __syncthreads();
+  u16* wm = s.mark + w * SB;
+  const int per = ((tot + NW - 1) / NW + WAVE - 1) & ~(WAVE - 1);
+  const int pb = w * per, pe = min(tot, pb + per);
+  if (pb < pe) {  // wave-uniform
+    int lo = 0, hi = nvalid - 1;  // last entry with ex <= pb: the owner of pb
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s.ex[mid] <= pb) lo = mid; else hi = mid - 1;
+    }
+    int carry = lo + 1;  // owner of the sub-batch's first product, plus one
+    for (int p0 = pb; p0 < pe; p0 += SB) {
+      const int p1 = min(pe, p0 + SB);
+      // Mark the nonempty entries that start in (p0, p1).
+      for (int c0 = carry; ; c0 += WAVE) {
+        const int c = c0 + lane;
+        const int xc = c < nvalid ? s.ex[c] : INT_MAX;
+        if (xc < p1 && s.ex[c + 1] > xc) wm[xc - p0] = (u16)(c + 1);
+        if (!__any(xc < p1)) break;
       }
-      int carry = lo + 1;
+      wave_sync();
       OffT idx[ROWS];
       float a[ROWS];
       bool ok[ROWS];
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
-        const int q = w * PW + r * WAVE + lane;
-        int m = s.mark[q];
-        s.mark[q] = 0;
+        const int q = r * WAVE + lane;
+        int m = wm[q];
+        wm[q] = 0;
         m = max(wave_max_scan(m), carry);
         carry = __builtin_amdgcn_readlane(m, WAVE - 1);
         const int e = m - 1;
-        ok[r] = p0 + q < tot;
+        ok[r] = p0 + q < p1;
         idx[r] = s.off[e] + (OffT)(p0 + q);
         if constexpr (VAL) a[r] = s.av[e];
       }
+      wave_sync();  // marks cleared before the next sub-batch writes them
       // Unconditional loads (lanes past the end read entry 0): a load under
       // a branch makes the compiler wait for every earlier load before the
       // next one, which would serialise the rows.
@@ -275,8 +302,8 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
       for (int r = 0; r < ROWS; ++r)
         if (ok[r]) f(c[r], VAL ? a[r] * v[r] : 0.f);
     }
-    __syncthreads();  // marks cleared before the next batch writes them
   }
+  __syncthreads();  // the chunk's entries are read before the next chunk writes them
   return tot;
 }
 
@@ -319,6 +346,31 @@ struct HashArgs {
   int bin;
 };
 
+// Minimum and maximum over a group of G lanes (G <= 64), or over the whole
+// workgroup (G == WG, through scr[WG/64]); every lane receives both.
+template <int G, int WG>
+__device__ __forceinline__ void group_minmax(i32& mn, i32& mx, i32* scr) {
+  constexpr int W = G <= WAVE ? G : WAVE;
+#pragma unroll
+  for (int d = W >> 1; d > 0; d >>= 1) {
+    mn = min(mn, __shfl_xor(mn, d, W));
+    mx = max(mx, __shfl_xor(mx, d, W));
+  }
+  if constexpr (G > WAVE) {
+    constexpr int NW = WG / WAVE;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    __syncthreads();
+    if (lane == 0) scr[w] = mn;
+    __syncthreads();
+    for (int i = 0; i < NW; ++i) mn = min(mn, scr[i]);
+    __syncthreads();
+    if (lane == 0) scr[w] = mx;
+    __syncthreads();
+    for (int i = 0; i < NW; ++i) mx = max(mx, scr[i]);
+    __syncthreads();
+  }
+}
+
 template <int G, int T, int WG, int MODE, bool BATCH>
 __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   static_assert(G <= WAVE || G == WG, "a multi-wave group must be the whole workgroup");
@@ -328,12 +380,22 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   constexpr int SPL = T / G;
   __shared__ __align__(16) i32 s_keys[R * T];
   __shared__ float s_vals[NUMERIC ? R * T : 1];
-  __shared__ i64 s_bs[WG];
-  __shared__ i32 s_ex[WG];
-  __shared__ float s_av[NUMERIC ? WG : 1];
+  // Per-entry chunk data: B start (8 B), first product (4 B), A value (4 B).
+  __shared__ __align__(16) unsigned char s_ent[WG * 16];
+  i64* s_bs = reinterpret_cast<i64*>(s_ent);
+  i32* s_ex = reinterpret_cast<i32*>(s_ent + 8 * WG);
+  float* s_av = reinterpret_cast<float*>(s_ent + 12 * WG);
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_cnt[MODE == MODE_LEARN ? R : 1];
   __shared__ i32 s_ovf[MODE == MODE_LEARN ? R : 1];
+  // Column bitmap of a numeric row (BMW words per row): the output order
+  // when the row's columns span fewer than 32*BMW.  Whole-workgroup rows of
+  // a large workgroup reuse s_ent, idle once the products are in the table.
+  constexpr bool BM_IN_ENT = G == WG && 4 * WG >= 2048;  // 65536 columns or more
+  constexpr int BMW = BM_IN_ENT ? 4 * WG : T;
+  constexpr int WPL = BMW / G;  // bitmap words per lane
+  static_assert(BMW % G == 0, "bitmap words split evenly over the group");
+  __shared__ u32 s_bm[NUMERIC && !BM_IN_ENT ? R * T : 1];
   const i32 nlist = a.dcount ? a.dcount[a.bin] : a.nlist;
   const i32* __restrict__ list = a.dcount ? a.list + a.doff[a.bin] : a.list;
   const i64* __restrict__ arp = a.arp;
@@ -501,7 +563,70 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
         return;
       }
     }
-    // Compact the occupied slots (packed column<<32 | value bits) ...
+    // Column range of the row.  When it is narrower than the bitmap, the
+    // row's columns are set in the bitmap and lane l writes the entries of
+    // words [l*WPL, (l+1)*WPL) in column order at their rank (prefix of
+    // set bits), reading each value back from the table: no sort.
+    i32 cmn = INT_MAX, cmx = -1;
+#pragma unroll
+    for (int m = 0; m < SPL; ++m) {
+      const i32 kk = keys[l + m * G];
+      if (kk != EMPTY_KEY) {
+        cmn = min(cmn, kk);
+        cmx = max(cmx, kk);
+      }
+    }
+    group_minmax<G, WG>(cmn, cmx, s_scr);
+    if (cmx >= 0 && (i64)cmx - cmn < (i64)BMW * 32) {  // uniform over the group
+      u32* bm = BM_IN_ENT ? reinterpret_cast<u32*>(s_ent) : s_bm + (BM_IN_ENT ? 0 : g * T);
+      for (int d = l; d < BMW; d += G) bm[d] = 0u;
+      group_sync<G>();
+#pragma unroll
+      for (int m = 0; m < SPL; ++m) {
+        const i32 kk = keys[l + m * G];
+        if (kk != EMPTY_KEY) {
+          const int x = kk - cmn;
+          lds_or_u32(&bm[x >> 5], 1u << (x & 31));
+        }
+      }
+      group_sync<G>();
+      u32 wv[WPL];
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < WPL; ++j) {
+        wv[j] = bm[l * WPL + j];
+        cnt += __popc(wv[j]);
+      }
+      int inc, N;
+      if constexpr (G <= WAVE) {
+        inc = seg_scan_incl<G>(cnt);
+        N = __shfl(inc, G - 1, G);
+      } else {
+        inc = block_scan_incl<WG>(cnt, s_scr, &N);
+      }
+      if (active) {
+        const i64 off = a.crp[row];
+        i64 q = off + (inc - cnt);
+#pragma unroll
+        for (int j = 0; j < WPL; ++j) {
+          u32 bits = wv[j];
+          while (bits) {
+            const i32 c = cmn + (l * WPL + j) * 32 + (__ffs(bits) - 1);
+            bits &= bits - 1;
+            u32 h = hash_col<LT>(c);
+            while (keys[h] != c) h = (h + 1) & (T - 1);
+            a.ccol[q] = c;
+            a.cval[q] = vals[h];
+            ++q;
+          }
+        }
+        if constexpr (MODE == MODE_LEARN) {
+          if (l == 0) a.nnz[row] = N;
+        }
+      }
+      return;
+    }
+    // Otherwise compact the occupied slots (packed column<<32 | value bits) ...
     u64 mine[SPL];
     int mc = 0;
 #pragma unroll
@@ -804,7 +929,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
   __shared__ float s_vals[TILE_W];
   __shared__ u32 s_bits[TILE_W / 32];
   __shared__ i64 s_off[WG];
-  __shared__ i32 s_ex[WG];
+  __shared__ i32 s_ex[WG + 1];
   __shared__ float s_av[WG];
   __shared__ u16 s_mark[4 * WG];
   __shared__ i32 s_scr[WG / WAVE];
@@ -996,6 +1121,42 @@ __device__ __forceinline__ void seg_words(const u32* bits, int t, int t1, bool d
   }
 }
 
+// Writes the n entries of a step from its staged words cw[0..nw) (prefix of
+// set bits in the high half, bits in the low half) at out..out+n-1 of C:
+// thread l takes the contiguous entries [l*K, l*K+K), K = ceil(n/WG), finds
+// the word of its first entry by one binary search and walks the words from
+// there.  Values come from acc[q] (compressed: accumulators in rank order)
+// or acc[column] (dense), which are re-zeroed.  Ends with a barrier.
+template <int WG, bool DENSE>
+__device__ __forceinline__ void emit_words(const u64* cw, int nw, int n, i64 out, i64 cbase,
+                                           float* acc, i32* __restrict__ ccol,
+                                           float* __restrict__ cval) {
+  const int K = (n + WG - 1) / WG;
+  int q = threadIdx.x * K;
+  const int qe = min(n, q + K);
+  if (q < qe) {
+    int lo = 0, hi = nw - 1;  // last word whose prefix is <= q: it holds entry q
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((int)(cw[mid] >> 32) <= q) lo = mid; else hi = mid - 1;
+    }
+    int wi = lo;
+    u64 wd = cw[wi];
+    u32 bits = (u32)wd;
+    for (int k = q - (int)(wd >> 32); k > 0; --k) bits &= bits - 1;
+    for (; q < qe; ++q) {
+      while (bits == 0) bits = (u32)cw[++wi];
+      const int x = wi * 32 + (__ffs(bits) - 1);
+      bits &= bits - 1;
+      const int ai = DENSE ? x : q;
+      ccol[out + q] = (i32)(cbase + x);
+      cval[out + q] = acc[ai];
+      acc[ai] = 0.f;
+    }
+  }
+  __syncthreads();
+}
+
 template <bool BATCH, class OffT, int HC, int ROWS>
 __global__ __launch_bounds__(1024) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
@@ -1010,8 +1171,8 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
   __shared__ __align__(16) unsigned char s_raw[HSEG_LDS];
   __shared__ OffT s_off[WG];
   __shared__ float s_av[WG];
-  __shared__ i32 s_ex[WG];
-  __shared__ u16 s_mark[ROWS * WG];
+  __shared__ i32 s_ex[WG + 1];
+  __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   float* s_acc = reinterpret_cast<float*>(s_raw);
@@ -1169,40 +1330,22 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
             chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_rank);
         }
       }
-      // Output: every thread writes the entries of its own words.
+      // Output, balanced over the threads.
       if (dense) {
+        // Stage the tile's words with their prefix in the mark array (idle
+        // between chunks), then restore its zeros.
         const u32 b = wv[0];
         const int pcb = (int)__popc(b);
         int tot;
         const int inc = block_scan_incl<WG>(pcb, s_scr, &tot);
-        i64 q = out + (inc - pcb);
-        u32 r = b;
-        while (r) {
-          const int x = l * 32 + (__ffs(r) - 1);
-          r &= r - 1;
-          ccol[q] = (i32)(cbase + x);
-          cval[q] = s_acc[x];
-          s_acc[x] = 0.f;
-          ++q;
-        }
-        __syncthreads();  // s_scr is reused by the next scan
+        u64* dw = reinterpret_cast<u64*>(s_mark);
+        dw[l] = ((u64)(u32)(inc - pcb) << 32) | b;
+        __syncthreads();
+        emit_words<WG, true>(dw, WG, n, out, cbase, s_acc, ccol, cval);
+        reinterpret_cast<u64*>(s_mark)[l] = 0ull;
+        __syncthreads();
       } else {
-        int q = wpfx;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          u32 r = wv[j];
-          const i64 cw = cbase + (i64)(l * 8 + j) * 32;
-          while (r) {
-            const int bit = __ffs(r) - 1;
-            r &= r - 1;
-            ccol[out + q] = (i32)(cw + bit);
-            cval[out + q] = s_acc[q];
-            s_acc[q] = 0.f;
-            ++q;
-          }
-        }
-        // No barrier: the next step rewrites only this thread's words, and
-        // its accumulation starts after the barriers of its staging scan.
+        emit_words<WG, false>(s_cw, (t1 - t) * (TILE_W / 32), n, out, cbase, s_acc, ccol, cval);
       }
       out += n;
       t = tn;
@@ -1393,7 +1536,7 @@ void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const 
   launch_hash<16, 128, 256, MODE, BATCH>(names[MODE][1], a, list + off[1], cnt[1], s);
   launch_hash<64, 512, 256, MODE, BATCH>(names[MODE][2], a, list + off[2], cnt[2], s);
   launch_hash<256, 2048, 256, MODE, BATCH>(names[MODE][3], a, list + off[3], cnt[3], s);
-  launch_hash<1024, 8192, 1024, MODE, BATCH>(names[MODE][4], a, list + off[4], cnt[4], s);
+  launch_hash<512, 8192, 512, MODE, BATCH>(names[MODE][4], a, list + off[4], cnt[4], s);
 }
 
 HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
@@ -1552,7 +1695,7 @@ void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipSt
   RT_HIP(hipMemsetAsync(tot.p, 0, 2 * sizeof(unsigned long long), s));
   if (m > 0) {
     ProfScope ps("row_products", s);
-    const dim3 grid((unsigned)((m * 16 + 255) / 256));
+    const dim3 grid(row_products_grid(m));
     if (bbase)
       hipLaunchKernelGGL(k_row_products<true>, grid, dim3(256), 0, s, m, A.rp, A.col, B.rp, bbase,
                          prod, tot.p);
@@ -1725,7 +1868,7 @@ void launch_hash_bins_dev(const HashArgs& a, const DevBins& b, i64 max_rows, hip
   launch_hash_dev<16, 128, 256, MODE, BATCH>(a, b, 1, max_rows, s);
   launch_hash_dev<64, 512, 256, MODE, BATCH>(a, b, 2, max_rows, s);
   launch_hash_dev<256, 2048, 256, MODE, BATCH>(a, b, 3, max_rows, s);
-  launch_hash_dev<1024, 8192, 1024, MODE, BATCH>(a, b, 4, max_rows, s);
+  launch_hash_dev<512, 8192, 512, MODE, BATCH>(a, b, 4, max_rows, s);
 }
 
 // Captures the launches `body` issues on `s` into a graph, runs it once and
@@ -1791,7 +1934,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(tot.p),
                          (i64)2, (i64)0);
       if (m > 0)
-        hipLaunchKernelGGL(k_row_products<BATCH>, dim3((unsigned)((m * 16 + 255) / 256)),
+        hipLaunchKernelGGL(k_row_products<BATCH>, dim3(row_products_grid(m)),
                            dim3(256), 0, s, m, A.rp, A.col, B.rp, bbase, prod.p, tot.p);
       hipLaunchKernelGGL(k_fill_i64, dim3(fill_grid(M)), dim3(256), 0, s, nnz.p, M, (i64)0);
       bin_rows_dev(prod.p, m, B.cols, tu.sym, sb, s);
