@@ -19,10 +19,13 @@ namespace rt {
 using namespace dev;
 
 constexpr int NBIN = 6;
-constexpr int TILE_W = 32768;                   // columns per dense numeric tile
-constexpr int WIN_TILES = 32;                   // tiles per symbolic bitmap window
+constexpr int TILE_W = 16384;                   // columns per dense numeric tile
+constexpr int WIN_TILES = 64;                   // tiles per symbolic bitmap window
 constexpr i64 WIN_W = (i64)TILE_W * WIN_TILES;  // 2^20 columns per window
 constexpr int UNR = 4;                          // products gathered per lane per step
+// Threads of the long-row numeric kernels: one presence word per thread of a
+// tile, and small enough LDS (< 80 KB) for two workgroups per CU.
+constexpr int HWG = TILE_W / 32;
 
 struct Thr {
   i64 t[5];
@@ -386,7 +389,6 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   i32* s_ex = reinterpret_cast<i32*>(s_ent + 8 * WG);
   float* s_av = reinterpret_cast<float*>(s_ent + 12 * WG);
   __shared__ i32 s_scr[WG / WAVE];
-  __shared__ i32 s_cnt[MODE == MODE_LEARN ? R : 1];
   __shared__ i32 s_ovf[MODE == MODE_LEARN ? R : 1];
   // Column bitmap of a numeric row (BMW words per row): the output order
   // when the row's columns span fewer than 32*BMW.  Whole-workgroup rows of
@@ -424,10 +426,7 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
     if constexpr (NUMERIC) vals[s] = 0.f;
   }
   if constexpr (MODE == MODE_LEARN) {
-    if (l == 0) {
-      s_cnt[g] = 0;
-      s_ovf[g] = 0;
-    }
+    if (l == 0) s_ovf[g] = 0;
   }
   group_sync<G>();
 
@@ -496,23 +495,20 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
         const i32 c = cu[u];
         u32 h = hash_col<LT>(c);
         if constexpr (MODE == MODE_LEARN) {
-          // Capacity-limited insert: the row is abandoned (and flagged) as
-          // soon as it holds more distinct columns than its predicted
-          // capacity; the probe count is bounded by T as well, so no lane
-          // can spin.
+          // Probe-limited insert: a row whose table gets crowded (a probe
+          // run of PLIM slots) is abandoned and flagged for the exact rerun,
+          // so no lane can spin; its distinct count is checked against the
+          // predicted capacity once the products are in.
           if (s_ovf[g]) continue;
+          constexpr int PLIM = T < 64 ? T : 64;
           const float v = avu[u] * bvu[u];
           bool ok = false;
-          for (int probe = 0; probe < T; ++probe) {
+          for (int probe = 0; probe < PLIM; ++probe) {
             i32 kk = keys[h];
             if (kk == c) { ok = true; break; }
             if (kk == EMPTY_KEY) {
               kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-              if (kk == EMPTY_KEY) {
-                if (atomicAdd(&s_cnt[g], 1) >= cap) s_ovf[g] = 1;
-                ok = true;
-                break;
-              }
+              if (kk == EMPTY_KEY) { ++mycnt; ok = true; break; }
               if (kk == c) { ok = true; break; }
             }
             h = (h + 1) & (T - 1);
@@ -555,7 +551,10 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   } else {
     if constexpr (MODE == MODE_LEARN) {
       // s_ovf is uniform over the group (and over the workgroup when G == WG).
-      if (s_ovf[g]) {
+      int distinct;
+      if constexpr (G <= WAVE) distinct = seg_reduce_sum<G>(mycnt);
+      else distinct = block_reduce_sum<WG>(mycnt, s_scr);
+      if (s_ovf[g] || distinct > cap) {
         if (active && l == 0) {
           a.ovf[row] = 1;
           a.nnz[row] = 0;
@@ -914,7 +913,7 @@ __device__ __forceinline__ int tile_emit(i64 out, i64 cbase, i64 limit, float* s
 //   is then abandoned).  No tile counts: a tile whose chunk scans total zero
 //   products is skipped.
 template <bool BATCH, bool LEARN>
-__global__ __launch_bounds__(1024) void k_num_heavy(
+__global__ __launch_bounds__(HWG, 4) void k_num_heavy(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const float* __restrict__ aval,
@@ -923,7 +922,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
     float* __restrict__ cval, const i64* __restrict__ cap, i64* __restrict__ nnz,
     i32* __restrict__ ovf, i32* __restrict__ next) {
-  constexpr int WG = 1024;
+  constexpr int WG = HWG;
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   __shared__ float s_vals[TILE_W];
@@ -1071,13 +1070,13 @@ __global__ __launch_bounds__(1024) void k_num_heavy(
 // bitmap words are loaded while the current step runs.  Longer rows re-read
 // their entries per step and chunk.
 constexpr int SEG_TILES = 8;
-constexpr int SEG_WORDS = SEG_TILES * (TILE_W / 32);  // 8192 bitmap words
-constexpr int SEG_CAP = 16384;                        // entries per compressed step
-// LDS map of the accumulator region (bytes): dense steps use [0, 128K) as
-// 32768 accumulators; compressed steps use [0, 64K) as accumulators and
-// [64K, 128K) as one u64 per bitmap word: the word's popcount prefix in the
+constexpr int SEG_WORDS = SEG_TILES * (TILE_W / 32);  // 4096 bitmap words
+constexpr int SEG_CAP = TILE_W / 2;                   // entries per compressed step
+// LDS map of the accumulator region (bytes): dense steps use [0, 64K) as
+// 16384 accumulators; compressed steps use [0, 32K) as accumulators and
+// [32K, 64K) as one u64 per bitmap word: the word's popcount prefix in the
 // high half, its bits in the low half.
-constexpr int HSEG_LDS = 128 * 1024;
+constexpr int HSEG_LDS = TILE_W * 4;
 
 // Step [t, t1) of a row from tile t (whose count c0 > 0): n entries, dense
 // when the tile alone exceeds SEG_CAP.
@@ -1122,43 +1121,61 @@ __device__ __forceinline__ void seg_words(const u32* bits, int t, int t1, bool d
 }
 
 // Writes the n entries of a step from its staged words cw[0..nw) (prefix of
-// set bits in the high half, bits in the low half) at out..out+n-1 of C:
-// thread l takes the contiguous entries [l*K, l*K+K), K = ceil(n/WG), finds
-// the word of its first entry by one binary search and walks the words from
-// there.  Values come from acc[q] (compressed: accumulators in rank order)
-// or acc[column] (dense), which are re-zeroed.  Ends with a barrier.
+// set bits in the high half, bits in the low half) at out..out+n-1 of C.
+// The work is a merge of two sequences, words and entries: word w is one
+// event at position w + prefix(w), followed by its entries.  Thread l takes
+// an equal share of the nw + n events (one binary search for its first
+// word, then a walk), so long runs of empty words are split as evenly as
+// dense ones.  Values come from acc[q] (compressed: accumulators in rank
+// order) or acc[column] (dense), which are re-zeroed.  Ends with a barrier.
 template <int WG, bool DENSE>
 __device__ __forceinline__ void emit_words(const u64* cw, int nw, int n, i64 out, i64 cbase,
                                            float* acc, i32* __restrict__ ccol,
                                            float* __restrict__ cval) {
-  const int K = (n + WG - 1) / WG;
-  int q = threadIdx.x * K;
-  const int qe = min(n, q + K);
-  if (q < qe) {
-    int lo = 0, hi = nw - 1;  // last word whose prefix is <= q: it holds entry q
+  const int E = nw + n;
+  const int per = (E + WG - 1) / WG;
+  int d = threadIdx.x * per;
+  const int de = min(E, d + per);
+  if (d < de) {
+    int lo = 0, hi = nw - 1;  // last word whose event position is <= d
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if ((int)(cw[mid] >> 32) <= q) lo = mid; else hi = mid - 1;
+      if (mid + (int)(cw[mid] >> 32) <= d) lo = mid; else hi = mid - 1;
     }
-    int wi = lo;
-    u64 wd = cw[wi];
+    int w = lo;
+    const u64 wd = cw[w];
     u32 bits = (u32)wd;
-    for (int k = q - (int)(wd >> 32); k > 0; --k) bits &= bits - 1;
-    for (; q < qe; ++q) {
-      while (bits == 0) bits = (u32)cw[++wi];
-      const int x = wi * 32 + (__ffs(bits) - 1);
+    int q = (int)(wd >> 32);
+    const int k = d - (w + q);  // 0: the word's own event; k >= 1: its entry k-1
+    if (k == 0) {
+      ++d;
+    } else {
+      for (int i = 1; i < k; ++i) bits &= bits - 1;
+      q += k - 1;
+    }
+    while (d < de) {
+      if (bits == 0) {  // the next word's event
+        if (++w >= nw) break;
+        bits = (u32)cw[w];
+        ++d;
+        continue;
+      }
+      const int x = w * 32 + (__ffs(bits) - 1);
       bits &= bits - 1;
       const int ai = DENSE ? x : q;
       ccol[out + q] = (i32)(cbase + x);
       cval[out + q] = acc[ai];
       acc[ai] = 0.f;
+      ++q;
+      ++d;
     }
   }
   __syncthreads();
 }
 
 template <bool BATCH, class OffT, int HC, int ROWS>
-__global__ __launch_bounds__(1024) void k_num_heavy_seg(
+// Two workgroups of HWG threads per CU: 16 waves, four per SIMD (<= 128 VGPRs).
+__global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const float* __restrict__ aval,
@@ -1166,8 +1183,8 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
     const float* __restrict__ bval, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
     float* __restrict__ cval, i32* __restrict__ next) {
-  constexpr int WG = 1024;
-  static_assert(TILE_W / 32 == WG && SEG_WORDS / WG == 8, "layout assumes 1024 threads");
+  constexpr int WG = HWG;
+  static_assert(TILE_W / 32 == WG && SEG_WORDS / WG == 8, "eight bitmap words per thread");
   __shared__ __align__(16) unsigned char s_raw[HSEG_LDS];
   __shared__ OffT s_off[WG];
   __shared__ float s_av[WG];
@@ -1176,7 +1193,7 @@ __global__ __launch_bounds__(1024) void k_num_heavy_seg(
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   float* s_acc = reinterpret_cast<float*>(s_raw);
-  u64* s_cw = reinterpret_cast<u64*>(s_raw + 64 * 1024);
+  u64* s_cw = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4);
   const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_scr};
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
@@ -1602,7 +1619,7 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
                          B.rows, heavy.nt, B.rp, B.col, slab.p);
       RT_HIP(hipGetLastError());
     }
-    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus);
+    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     DBuf<i32> next(1);
     zero_slot(next.p, s);
     {
@@ -1611,15 +1628,15 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
       // int32 B offsets (and eight products per lane in flight) when B's
       // entries are indexable by int32.
       if (heavy.rowbits.p && B.nnz < ((i64)1 << 31))
-        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 2, 8>), dim3(grid), dim3(1024), 0, s, rs,
+        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 2, 8>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
                            A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, next.p);
       else if (heavy.rowbits.p)
-        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 4, 4>), dim3(grid), dim3(1024), 0, s, rs,
+        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 4, 4>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
                            A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, next.p);
       else
-        hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(1024), 0, s, rs,
+        hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val, B.rp,
                            B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr,
                            nullptr, next.p);
@@ -1750,18 +1767,18 @@ void learned_heavy_pass(const Csr& A, const Csr& B, const i64* bbase, const i64*
                        B.rows, nt, B.rp, B.col, slab.p);
     RT_HIP(hipGetLastError());
   }
-  const unsigned grid = (unsigned)std::min<i64>(cnt[5], (i64)device().cus);
+  const unsigned grid = (unsigned)std::min<i64>(cnt[5], (i64)device().cus * 2);
   DBuf<i32> next(1);
   zero_slot(next.p, s);
   {
     ProfScope ps("learn_heavy", s);
     if (bbase)
-      hipLaunchKernelGGL((k_num_heavy<true, true>), dim3(grid), dim3(1024), 0, s,
+      hipLaunchKernelGGL((k_num_heavy<true, true>), dim3(grid), dim3(HWG), 0, s,
                          RowSet{list.p + off[5], cnt[5], nullptr, nullptr, 0}, nt, nullptr, key,
                          (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p, tmp_off,
                          tcol, tval, cap, nnz, ovf, next.p);
     else
-      hipLaunchKernelGGL((k_num_heavy<false, true>), dim3(grid), dim3(1024), 0, s,
+      hipLaunchKernelGGL((k_num_heavy<false, true>), dim3(grid), dim3(HWG), 0, s,
                          RowSet{list.p + off[5], cnt[5], nullptr, nullptr, 0}, nt, nullptr, key,
                          (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, nullptr, slab.p, tmp_off,
                          tcol, tval, cap, nnz, ovf, next.p);
@@ -1857,7 +1874,18 @@ void launch_hash_dev(HashArgs a, const DevBins& b, int bin, i64 max_rows, hipStr
   a.doff = b.off.p;
   a.bin = bin;
   const i64 groups = std::max<i64>(1, (max_rows + R - 1) / R);
-  const unsigned grid = (unsigned)std::min<i64>(groups, (i64)device().cus * (2048 / WG));
+  // Exactly the workgroups the device holds at once, so the grid-stride
+  // loop runs in a single round on every CU.
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hash<G, T, WG, MODE, BATCH>, WG, 0) !=
+            hipSuccess ||
+        n < 1)
+      n = 1;
+    per_cu = n;
+  }
+  const unsigned grid = (unsigned)std::min<i64>(groups, (i64)device().cus * per_cu);
   hipLaunchKernelGGL((k_hash<G, T, WG, MODE, BATCH>), dim3(grid), dim3(WG), 0, s, a);
   RT_HIP(hipGetLastError());
 }
@@ -1963,7 +1991,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
                            s, B.rows, nt, B.rp, B.col, slab.p);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p + 1, s);
-      hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(heavy_grid), dim3(1024), 0, s, heavy,
+      hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(heavy_grid * 2), dim3(HWG), 0, s, heavy,
                          nt, tilecnt.p, nnz.p, tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val,
                          bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr, nullptr, next.p + 1);
       HashArgs a = base;
