@@ -1120,61 +1120,87 @@ __device__ __forceinline__ void seg_words(const u32* bits, int t, int t1, bool d
   }
 }
 
-// Writes the n entries of a step from its staged words cw[0..nw) (prefix of
-// set bits in the high half, bits in the low half) at out..out+n-1 of C.
-// The work is a merge of two sequences, words and entries: word w is one
-// event at position w + prefix(w), followed by its entries.  Thread l takes
-// an equal share of the nw + n events (one binary search for its first
-// word, then a walk), so long runs of empty words are split as evenly as
-// dense ones.  Values come from acc[q] (compressed: accumulators in rank
-// order) or acc[column] (dense), which are re-zeroed.  Ends with a barrier.
-template <int WG, bool DENSE>
-__device__ __forceinline__ void emit_words(const u64* cw, int nw, int n, i64 out, i64 cbase,
-                                           float* acc, i32* __restrict__ ccol,
-                                           float* __restrict__ cval) {
+// The entries of a step, in column order, from its staged words cw[0..nw)
+// (prefix of set bits in the high half, bits in the low half).  The work is
+// a merge of two sequences, words and entries: word w is one event at
+// position w + prefix(w), followed by its entries.  Thread l takes an equal
+// share of the nw + n events (one binary search for its first word, then a
+// walk), so long runs of empty words are split as evenly as dense ones; f(q,
+// x) is called for entry q at column offset x.
+template <int WG, class F>
+__device__ __forceinline__ void walk_words(const u64* cw, int nw, int n, F&& f) {
   const int E = nw + n;
   const int per = (E + WG - 1) / WG;
   int d = threadIdx.x * per;
   const int de = min(E, d + per);
-  if (d < de) {
-    int lo = 0, hi = nw - 1;  // last word whose event position is <= d
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (mid + (int)(cw[mid] >> 32) <= d) lo = mid; else hi = mid - 1;
-    }
-    int w = lo;
-    const u64 wd = cw[w];
-    u32 bits = (u32)wd;
-    int q = (int)(wd >> 32);
-    const int k = d - (w + q);  // 0: the word's own event; k >= 1: its entry k-1
-    if (k == 0) {
+  if (d >= de) return;
+  int lo = 0, hi = nw - 1;  // last word whose event position is <= d
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mid + (int)(cw[mid] >> 32) <= d) lo = mid; else hi = mid - 1;
+  }
+  int w = lo;
+  const u64 wd = cw[w];
+  u32 bits = (u32)wd;
+  int q = (int)(wd >> 32);
+  const int k = d - (w + q);  // 0: the word's own event; k >= 1: its entry k-1
+  if (k == 0) {
+    ++d;
+  } else {
+    for (int i = 1; i < k; ++i) bits &= bits - 1;
+    q += k - 1;
+  }
+  while (d < de) {
+    if (bits == 0) {  // the next word's event
+      if (++w >= nw) break;
+      bits = (u32)cw[w];
       ++d;
-    } else {
-      for (int i = 1; i < k; ++i) bits &= bits - 1;
-      q += k - 1;
+      continue;
     }
-    while (d < de) {
-      if (bits == 0) {  // the next word's event
-        if (++w >= nw) break;
-        bits = (u32)cw[w];
-        ++d;
-        continue;
-      }
-      const int x = w * 32 + (__ffs(bits) - 1);
-      bits &= bits - 1;
-      const int ai = DENSE ? x : q;
-      ccol[out + q] = (i32)(cbase + x);
-      cval[out + q] = acc[ai];
-      acc[ai] = 0.f;
-      ++q;
-      ++d;
-    }
+    f(q, w * 32 + (__ffs(bits) - 1));
+    bits &= bits - 1;
+    ++q;
+    ++d;
+  }
+}
+
+// Writes a compressed step (accumulators acc[0..n) in rank order) at
+// out..out+n-1 of C with coalesced stores: the values first, then the walk
+// turns acc into the column offsets, which are written and the array
+// re-zeroed.  Three barriers.
+template <int WG>
+__device__ __forceinline__ void emit_ranked(const u64* cw, int nw, int n, i64 out, i64 cbase,
+                                            float* acc, i32* __restrict__ ccol,
+                                            float* __restrict__ cval) {
+  const int l = threadIdx.x;
+  for (int q = l; q < n; q += WG) cval[out + q] = acc[q];
+  __syncthreads();
+  i32* colx = reinterpret_cast<i32*>(acc);
+  walk_words<WG>(cw, nw, n, [&](int q, int x) { colx[q] = x; });
+  __syncthreads();
+  for (int q = l; q < n; q += WG) {
+    ccol[out + q] = (i32)(cbase + colx[q]);
+    colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
   }
   __syncthreads();
 }
 
-template <bool BATCH, class OffT, int HC, int ROWS>
+// Writes a dense step (accumulators acc[column]) at out..out+n-1 of C, each
+// thread storing the entries of its share of the walk, and re-zeroes them.
+template <int WG>
+__device__ __forceinline__ void emit_dense(const u64* cw, int nw, int n, i64 out, i64 cbase,
+                                           float* acc, i32* __restrict__ ccol,
+                                           float* __restrict__ cval) {
+  walk_words<WG>(cw, nw, n, [&](int q, int x) {
+    ccol[out + q] = (i32)(cbase + x);
+    cval[out + q] = acc[x];
+    acc[x] = 0.f;
+  });
+  __syncthreads();
+}
+
 // Two workgroups of HWG threads per CU: 16 waves, four per SIMD (<= 128 VGPRs).
+template <bool BATCH, class OffT, int HC, int ROWS>
 __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
@@ -1358,11 +1384,11 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         u64* dw = reinterpret_cast<u64*>(s_mark);
         dw[l] = ((u64)(u32)(inc - pcb) << 32) | b;
         __syncthreads();
-        emit_words<WG, true>(dw, WG, n, out, cbase, s_acc, ccol, cval);
+        emit_dense<WG>(dw, WG, n, out, cbase, s_acc, ccol, cval);
         reinterpret_cast<u64*>(s_mark)[l] = 0ull;
         __syncthreads();
       } else {
-        emit_words<WG, false>(s_cw, (t1 - t) * (TILE_W / 32), n, out, cbase, s_acc, ccol, cval);
+        emit_ranked<WG>(s_cw, (t1 - t) * (TILE_W / 32), n, out, cbase, s_acc, ccol, cval);
       }
       out += n;
       t = tn;
