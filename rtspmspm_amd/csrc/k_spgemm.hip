@@ -22,7 +22,7 @@ constexpr int NBIN = 6;
 constexpr int TILE_W = 16384;                   // columns per dense numeric tile
 constexpr int WIN_TILES = 64;                   // tiles per symbolic bitmap window
 constexpr i64 WIN_W = (i64)TILE_W * WIN_TILES;  // 2^20 columns per window
-constexpr int UNR = 4;                          // products gathered per lane per step
+constexpr int UNR = 8;                          // products gathered per lane per step
 // Threads of the long-row numeric kernels: one presence word per thread of a
 // tile, and small enough LDS (< 80 KB) for two workgroups per CU.
 constexpr int HWG = TILE_W / 32;
@@ -1120,82 +1120,38 @@ __device__ __forceinline__ void seg_words(const u32* bits, int t, int t1, bool d
   }
 }
 
-// The entries of a step, in column order, from its staged words cw[0..nw)
-// (prefix of set bits in the high half, bits in the low half).  The work is
-// a merge of two sequences, words and entries: word w is one event at
-// position w + prefix(w), followed by its entries.  Thread l takes an equal
-// share of the nw + n events (one binary search for its first word, then a
-// walk), so long runs of empty words are split as evenly as dense ones; f(q,
-// x) is called for entry q at column offset x.
-template <int WG, class F>
-__device__ __forceinline__ void walk_words(const u64* cw, int nw, int n, F&& f) {
-  const int E = nw + n;
-  const int per = (E + WG - 1) / WG;
-  int d = threadIdx.x * per;
-  const int de = min(E, d + per);
-  if (d >= de) return;
-  int lo = 0, hi = nw - 1;  // last word whose event position is <= d
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (mid + (int)(cw[mid] >> 32) <= d) lo = mid; else hi = mid - 1;
-  }
-  int w = lo;
-  const u64 wd = cw[w];
-  u32 bits = (u32)wd;
-  int q = (int)(wd >> 32);
-  const int k = d - (w + q);  // 0: the word's own event; k >= 1: its entry k-1
-  if (k == 0) {
-    ++d;
-  } else {
-    for (int i = 1; i < k; ++i) bits &= bits - 1;
-    q += k - 1;
-  }
-  while (d < de) {
-    if (bits == 0) {  // the next word's event
-      if (++w >= nw) break;
-      bits = (u32)cw[w];
-      ++d;
-      continue;
-    }
-    f(q, w * 32 + (__ffs(bits) - 1));
-    bits &= bits - 1;
-    ++q;
-    ++d;
-  }
-}
-
 // Writes a compressed step (accumulators acc[0..n) in rank order) at
-// out..out+n-1 of C with coalesced stores: the values first, then the walk
-// turns acc into the column offsets, which are written and the array
-// re-zeroed.  Three barriers.
+// out..out+n-1 of C with coalesced stores: the values first; then every
+// thread turns the accumulators of its own eight staged words (read with
+// independent loads: no chain of dependent LDS reads, which stall behind the
+// other workgroup's LDS traffic) into column offsets; these are written and
+// the array re-zeroed.  Three barriers.
 template <int WG>
-__device__ __forceinline__ void emit_ranked(const u64* cw, int nw, int n, i64 out, i64 cbase,
+__device__ __forceinline__ void emit_ranked(const u64* cw, int n, i64 out, i64 cbase,
                                             float* acc, i32* __restrict__ ccol,
                                             float* __restrict__ cval) {
   const int l = threadIdx.x;
   for (int q = l; q < n; q += WG) cval[out + q] = acc[q];
   __syncthreads();
   i32* colx = reinterpret_cast<i32*>(acc);
-  walk_words<WG>(cw, nw, n, [&](int q, int x) { colx[q] = x; });
+  u64 mw[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) mw[j] = cw[l * 8 + j];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    u32 bits = (u32)mw[j];
+    int q = (int)(mw[j] >> 32);
+    const int x0 = (l * 8 + j) * 32;
+    while (bits) {
+      colx[q++] = x0 + (__ffs(bits) - 1);
+      bits &= bits - 1;
+    }
+  }
   __syncthreads();
   for (int q = l; q < n; q += WG) {
     ccol[out + q] = (i32)(cbase + colx[q]);
     colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
   }
-  __syncthreads();
-}
-
-// Writes a dense step (accumulators acc[column]) at out..out+n-1 of C, each
-// thread storing the entries of its share of the walk, and re-zeroes them.
-template <int WG>
-__device__ __forceinline__ void emit_dense(const u64* cw, int nw, int n, i64 out, i64 cbase,
-                                           float* acc, i32* __restrict__ ccol,
-                                           float* __restrict__ cval) {
-  walk_words<WG>(cw, nw, n, [&](int q, int x) {
-    ccol[out + q] = (i32)(cbase + x);
-    cval[out + q] = acc[x];
-    acc[x] = 0.f;
-  });
   __syncthreads();
 }
 
@@ -1373,22 +1329,26 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
             chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_rank);
         }
       }
-      // Output, balanced over the threads.
+      // Output.  Dense tile: thread l writes the entries of its own word
+      // (words of a dense tile carry similar counts) at the word's prefix.
       if (dense) {
-        // Stage the tile's words with their prefix in the mark array (idle
-        // between chunks), then restore its zeros.
         const u32 b = wv[0];
         const int pcb = (int)__popc(b);
         int tot;
         const int inc = block_scan_incl<WG>(pcb, s_scr, &tot);
-        u64* dw = reinterpret_cast<u64*>(s_mark);
-        dw[l] = ((u64)(u32)(inc - pcb) << 32) | b;
-        __syncthreads();
-        emit_dense<WG>(dw, WG, n, out, cbase, s_acc, ccol, cval);
-        reinterpret_cast<u64*>(s_mark)[l] = 0ull;
-        __syncthreads();
+        i64 q = out + (inc - pcb);
+        u32 r = b;
+        while (r) {
+          const int x = l * 32 + (__ffs(r) - 1);
+          r &= r - 1;
+          ccol[q] = (i32)(cbase + x);
+          cval[q] = s_acc[x];
+          s_acc[x] = 0.f;
+          ++q;
+        }
+        __syncthreads();  // s_scr is reused by the next scan
       } else {
-        emit_ranked<WG>(s_cw, (t1 - t) * (TILE_W / 32), n, out, cbase, s_acc, ccol, cval);
+        emit_ranked<WG>(s_cw, n, out, cbase, s_acc, ccol, cval);
       }
       out += n;
       t = tn;
