@@ -20,8 +20,6 @@ using namespace dev;
 
 constexpr int NBIN = 6;
 constexpr int TILE_W = 16384;                   // columns per dense numeric tile
-constexpr int WIN_TILES = 64;                   // tiles per symbolic bitmap window
-constexpr i64 WIN_W = (i64)TILE_W * WIN_TILES;  // 2^20 columns per window
 constexpr int UNR = 8;                          // products gathered per lane per step
 // Threads of the long-row numeric kernels: one presence word per thread of a
 // tile, and small enough LDS (< 80 KB) for two workgroups per CU.
@@ -698,21 +696,28 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
   return *s_slot;
 }
 
-// LDS-bitmap symbolic kernel for long rows: one workgroup per row, one bit
-// per output column over a window of 2^20 columns (128 KB of LDS); wider
-// matrices take several windows.  Also records the nnz of every 32768-column
-// tile of the row, which positions the dense-tile numeric kernel's output.
+// LDS-bitmap symbolic kernel for long rows: one workgroup of HWG threads per
+// row, two resident per CU, one bit per output column over a window of
+// SYM_WIN_TILES tiles (2^19 columns, 64 KB of LDS); wider matrices take
+// several windows, and the slab pointers of B restrict every window to its
+// own products, so each product is gathered once.  Records the nnz of every
+// tile of the row (the long-row numeric kernels' step plan and output
+// offsets) and, when rowbits is set, the row's bitmap.
+constexpr int SYM_WIN_TILES = 32;
+constexpr int SYM_WIN = SYM_WIN_TILES * TILE_W;
+
 template <bool BATCH>
-__global__ __launch_bounds__(1024) void k_sym_bitmap(
+__global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
     RowSet rs, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const i64* __restrict__ brp,
-    const i32* __restrict__ bcol, const i64* __restrict__ bbase, i64 ncols, i32 nt,
-    i32* __restrict__ tilecnt, i64* __restrict__ out_nnz, u32* __restrict__ rowbits,
-    i32* __restrict__ next) {
-  constexpr int WG = 1024;
+    const i32* __restrict__ bcol, const i64* __restrict__ bbase, const i32* __restrict__ slab,
+    i64 ncols, i32 nt, i32* __restrict__ tilecnt, i64* __restrict__ out_nnz,
+    u32* __restrict__ rowbits, i32* __restrict__ next) {
+  constexpr int WG = HWG;
+  constexpr int WW = SYM_WIN / 32;  // bitmap words per window
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
-  __shared__ u32 s_bits[WIN_W / 32];
+  __shared__ u32 s_bits[WW];
   __shared__ i64 s_bs[WG];
   __shared__ i32 s_ex[WG];
   __shared__ i32 s_scr[WG / WAVE];
@@ -725,8 +730,10 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
     const i64 a0 = arp[row], a1 = arp[row + 1];
     const i64 bb = BATCH ? bbase[row] : 0;
     i64 total = 0;
-    for (i64 win = 0; win < ncols; win += WIN_W) {
-      const i64 wcols = min(WIN_W, ncols - win);
+    for (int tw0 = 0; tw0 < nt; tw0 += SYM_WIN_TILES) {
+      const int tw1 = min(nt, tw0 + SYM_WIN_TILES);
+      const i64 win = (i64)tw0 * TILE_W;
+      const i64 wcols = min((i64)SYM_WIN, ncols - win);
       const int nd = (int)((wcols + 31) / 32);
       for (int d = l; d < nd; d += WG) s_bits[d] = 0;
       __syncthreads();
@@ -736,8 +743,10 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
         i64 bs = 0;
         if (e < a1) {
           const i64 k = (i64)acol[e] + bb;
-          bs = brp[k];
-          len = (int)(brp[k + 1] - bs);
+          const i32* sk = slab + k * (nt + 1);
+          const i32 s0 = sk[tw0];
+          bs = brp[k] + s0;
+          len = sk[tw1] - s0;
         }
         int tot;
         const int inc = block_scan_incl<WG>(len, s_scr, &tot);
@@ -763,13 +772,13 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
 #pragma unroll
           for (int u = 0; u < UNR; ++u)
             if (idx[u] >= 0) {
-              const i64 c = (i64)cc[u] - win;
-              if (c >= 0 && c < wcols) lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
+              const int c = (int)((i64)cc[u] - win);  // in [0, wcols) by the slab bounds
+              lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
             }
         }
         __syncthreads();
       }
-      const int ntw = (int)((wcols + TILE_W - 1) / TILE_W);
+      const int ntw = tw1 - tw0;
       if (rowbits) {  // keep the row's column set (padded to whole tiles) for the numeric pass
         u32* dst = rowbits + it * ((i64)nt * (TILE_W / 32)) + win / 32;
         for (int d = l; d < ntw * (TILE_W / 32); d += WG) dst[d] = d < nd ? s_bits[d] : 0u;
@@ -780,7 +789,7 @@ __global__ __launch_bounds__(1024) void k_sym_bitmap(
         const int dend = min((t + 1) * (TILE_W / 32), nd);
         for (int d = t * (TILE_W / 32) + lane; d < dend; d += WAVE) c += __popc(s_bits[d]);
         c = seg_reduce_sum<WAVE>(c);
-        if (lane == 0) tilecnt[it * nt + (win / TILE_W) + t] = c;
+        if (lane == 0) tilecnt[it * nt + tw0 + t] = c;
         wsum += c;
       }
       total += block_reduce_sum<WG>(lane == 0 ? wsum : (i64)0, s_scr64);
@@ -1576,14 +1585,23 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     if (bm_bytes <= free_b / 3 + pool_cached_bytes() / 3)
       heavy.rowbits.alloc(bm_bytes / sizeof(u32));
-    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus);
+    // Tile boundaries of every B row (kept for the numeric pass).
+    heavy.slab.alloc((size_t)B.rows * (heavy.nt + 1));
+    if (B.rows > 0) {
+      const i64 nthreads = B.rows * (i64)(heavy.nt + 1);
+      ProfScope ps("slab_ptrs", s);
+      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
+                         B.rows, heavy.nt, B.rp, B.col, heavy.slab.p);
+      RT_HIP(hipGetLastError());
+    }
+    const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     heavy.next.alloc(1);
     zero_slot(heavy.next.p, s);
     ProfScope ps("sym_bitmap", s);
     const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
-    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(1024), 0, s, rs, A.rp, A.col, B.rp,
-                       B.col, bbase, B.cols, heavy.nt, heavy.tilecnt.p, nnz, heavy.rowbits.p,
-                       heavy.next.p);
+    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(HWG), 0, s, rs, A.rp, A.col, B.rp,
+                       B.col, bbase, heavy.slab.p, B.cols, heavy.nt, heavy.tilecnt.p, nnz,
+                       heavy.rowbits.p, heavy.next.p);
     RT_HIP(hipGetLastError());
   }
 }
@@ -1596,15 +1614,9 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
   int cnt[NBIN], off[NBIN + 1];
   bin_rows(key, m, LLONG_MAX, tu.num, list.p, cnt, off, s);
   if (cnt[5] > 0) {
-    RT_CHECK(heavy.n > 0, RT_E_INVALID, "tuning: numeric tile rows without symbolic tile counts");
-    DBuf<i32> slab((size_t)B.rows * (heavy.nt + 1));
-    {
-      const i64 nthreads = B.rows * (i64)(heavy.nt + 1);
-      ProfScope ps("slab_ptrs", s);
-      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
-                         B.rows, heavy.nt, B.rp, B.col, slab.p);
-      RT_HIP(hipGetLastError());
-    }
+    RT_CHECK(heavy.n > 0 && heavy.slab.p, RT_E_INVALID,
+             "tuning: numeric tile rows without symbolic tile counts");
+    const i32* const slab = heavy.slab.p;
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     DBuf<i32> next(1);
     zero_slot(next.p, s);
@@ -1616,15 +1628,15 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
       if (heavy.rowbits.p && B.nnz < ((i64)1 << 31))
         hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 2, 8>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
-                           A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, next.p);
+                           A.val, B.rp, B.col, B.val, bbase, slab, C.rp, C.col, C.val, next.p);
       else if (heavy.rowbits.p)
         hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 4, 4>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
-                           A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, next.p);
+                           A.val, B.rp, B.col, B.val, bbase, slab, C.rp, C.col, C.val, next.p);
       else
         hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val, B.rp,
-                           B.col, B.val, bbase, slab.p, C.rp, C.col, C.val, nullptr, nullptr,
+                           B.col, B.val, bbase, slab, C.rp, C.col, C.val, nullptr, nullptr,
                            nullptr, next.p);
       RT_HIP(hipGetLastError());
     }
@@ -1955,10 +1967,15 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       HashArgs a = base;
       a.nnz = nnz.p;
       launch_hash_bins_dev<MODE_SYM, BATCH>(a, sb, m, s);
+      if (B.rows > 0)
+        hipLaunchKernelGGL(k_slab_ptrs,
+                           dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)), dim3(256), 0,
+                           s, B.rows, nt, B.rp, B.col, slab.p);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p, s);
-      hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(heavy_grid), dim3(1024), 0, s, heavy, A.rp,
-                         A.col, B.rp, B.col, bbase, B.cols, nt, tilecnt.p, nnz.p, nullptr, next.p);
+      hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(heavy_grid * 2), dim3(HWG), 0, s, heavy, A.rp,
+                         A.col, B.rp, B.col, bbase, slab.p, B.cols, nt, tilecnt.p, nnz.p, nullptr,
+                         next.p);
       hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
       hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nbs, C.rp + m);
       if (m > 0)
@@ -1971,10 +1988,6 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
     C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
     ms_b = run_captured(s, [&] {
       bin_rows_dev(nnz.p, m, LLONG_MAX, tu.num, nb, s);
-      if (B.rows > 0)
-        hipLaunchKernelGGL(k_slab_ptrs,
-                           dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)), dim3(256), 0,
-                           s, B.rows, nt, B.rp, B.col, slab.p);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p + 1, s);
       hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(heavy_grid * 2), dim3(HWG), 0, s, heavy,

@@ -56,6 +56,7 @@ struct HeavyRows {
   DBuf<i32> tilecnt;  // n x nt
   DBuf<u32> rowbits;  // n x nt x 1024 words: each long row's column set (may be empty)
   DBuf<i32> next;     // row-slot counter of the symbolic long-row kernel
+  DBuf<i32> slab;     // B.rows x (nt+1) tile boundaries of B's rows
 };
 
 // Pipeline pieces.  `key` arrays select and size rows: rows with key 0 are
