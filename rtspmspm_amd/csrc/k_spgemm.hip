@@ -1060,35 +1060,39 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
 // Segmented long-row numeric kernel (exact method, row bitmaps available).
 //
 // The symbolic pass stored each long row's column set as a bitmap.  With it
-// the row is processed in few steps: a run of up to SEG_TILES consecutive
-// tiles holding at most SEG_CAP entries is one "compressed" step -- its
-// bitmap words and their popcount prefix are staged in LDS, every product is
-// added at its rank (prefix + popcount of the lower bits of its word), so the
-// accumulators already sit in column order.  A tile holding more than
-// SEG_CAP entries is one "dense" step: 32768 LDS accumulators indexed by
-// column.  Either way thread l owns bitmap words of the step (eight of a
-// compressed step, one of a dense tile) and writes their entries to C itself,
-// at the word's prefix: the output needs no search and no sort.  One
-// workgroup per row, rows grid-stride.
+// the row is processed in few steps.  A "compressed" step covers up to
+// SEG_MAXT consecutive tiles (2^20 columns) holding at most SEG_CAP entries
+// in at most SEG_MAXW nonzero bitmap words: only the window's nonzero words
+// are staged in LDS (their entry prefix, word index and bits), with a table
+// of 32-word groups (the slot of the group's first nonzero word and a mask
+// of its nonzero words).  A product's rank is then two LDS reads away
+// (group -> word), so the accumulators sit in column order.  A tile holding
+// more than SEG_CAP entries is one "dense" step: 16384 LDS accumulators
+// indexed by column.  Output needs no search and no sort.  Steps end only on
+// entry or word capacity, so a sparse long row takes one or two steps.
 //
-// Rows of up to HC*1024 entries keep their entries in registers for the
-// whole row: B row start, slab row, A value, and a cursor into the B row
-// that advances from step to step (a step's B range of entry k ends at
+// Rows of up to HC*WG entries keep their entries in registers for the whole
+// row: B row start, slab row, A value, and a cursor into the B row that
+// advances from step to step (a step's B range of entry k ends at
 // slab[k][t1], and the next step starts there, because the tiles skipped in
-// between hold no product of the row).  The next step's range ends and
-// bitmap words are loaded while the current step runs.  Longer rows re-read
-// their entries per step and chunk.
-constexpr int SEG_TILES = 8;
-constexpr int SEG_WORDS = SEG_TILES * (TILE_W / 32);  // 4096 bitmap words
-constexpr int SEG_CAP = TILE_W / 2;                   // entries per compressed step
-// LDS map of the accumulator region (bytes): dense steps use [0, 64K) as
-// 16384 accumulators; compressed steps use [0, 32K) as accumulators and
-// [32K, 64K) as one u64 per bitmap word: the word's popcount prefix in the
-// high half, its bits in the low half.
+// between hold no product of the row).  The next step's range ends are
+// loaded while the current step runs.  Longer rows re-read their entries per
+// step and chunk.
+constexpr int SEG_MAXT = 64;                                 // tiles per compressed window
+constexpr int SEG_CAP = 7168;                                // entries per compressed step
+constexpr int SEG_MAXW = 3584;                               // nonzero words per compressed step
+constexpr int SEG_GROUPS = SEG_MAXT * (TILE_W / 32) / 32;    // 32-word groups per window
+// LDS map of the accumulator region (bytes): dense steps use it whole as
+// 16384 accumulators; compressed steps use [0, 28K) as SEG_CAP accumulators,
+// [28K, 56K) as SEG_MAXW staged words (u64: entry prefix << 47 | word index
+// << 32 | bits) and [56K, 64K) as the group table (u64: first slot << 32 |
+// nonzero-word mask).
 constexpr int HSEG_LDS = TILE_W * 4;
+static_assert(SEG_CAP * 4 + SEG_MAXW * 8 + SEG_GROUPS * 8 <= HSEG_LDS, "accumulator region");
+static_assert(SEG_MAXT * TILE_W <= (1 << 20) && SEG_CAP < (1 << 13), "staged word fields");
 
 // Step [t, t1) of a row from tile t (whose count c0 > 0): n entries, dense
-// when the tile alone exceeds SEG_CAP.
+// when the tile alone exceeds SEG_CAP.  Trailing empty tiles are not kept.
 __device__ __forceinline__ void seg_step(const i32* s_tc, const i32* tc, int nt, int t, int& t1,
                                          int& n, bool& dense) {
   const int c0 = t < TC_LDS ? s_tc[t] : tc[t];
@@ -1096,12 +1100,18 @@ __device__ __forceinline__ void seg_step(const i32* s_tc, const i32* tc, int nt,
   n = c0;
   dense = c0 > SEG_CAP;
   if (!dense) {
-    while (t1 < nt && t1 - t < SEG_TILES) {
+    int words = min(c0, TILE_W / 32);  // a bound on the nonzero words
+    int tend = t1;
+    while (t1 < nt && t1 - t < SEG_MAXT) {
       const int ct = t1 < TC_LDS ? s_tc[t1] : tc[t1];
-      if (ct > SEG_CAP || n + ct > SEG_CAP) break;
+      const int wt = min(ct, TILE_W / 32);
+      if (ct > SEG_CAP || n + ct > SEG_CAP || words + wt > SEG_MAXW) break;
       n += ct;
+      words += wt;
       ++t1;
+      if (ct > 0) tend = t1;
     }
+    t1 = tend;
   }
 }
 
@@ -1110,47 +1120,77 @@ __device__ __forceinline__ int next_nonempty(const i32* s_tc, const i32* tc, int
   return t;
 }
 
-// This thread's bitmap words of step [t, t1): words 8l..8l+7 of a compressed
-// step, word l of a dense tile (in w[0]).
-__device__ __forceinline__ void seg_words(const u32* bits, int t, int t1, bool dense, u32 (&w)[8]) {
+// Stages the nonzero words of compressed step [t, t1) (see the LDS map):
+// thread l takes 32-word group l (and l + WG), counts its nonzero words and
+// entries, one block scan per round of WG groups gives the group's first
+// slot and entry prefix, and the group's words are written to their slots.
+// Returns the number of staged words.  Ends with a barrier.
+template <int WG>
+__device__ __forceinline__ int stage_words(const u32* __restrict__ src, int ngroups, u64* s_cwd,
+                                           u64* s_gt, u64* scr) {
   const int l = threadIdx.x;
-  const u32* src = bits + (i64)t * (TILE_W / 32);
-  if (dense) {
-    w[0] = src[l];
+  u64 carry = 0;  // (words << 32) | entries before this round
+  for (int g0 = 0; g0 < ngroups; g0 += WG) {
+    const int g = g0 + l;
+    u32 wv[32];
+    u32 mask = 0;
+    int ent = 0;
+    if (g < ngroups) {
+      const uint4* q = reinterpret_cast<const uint4*>(src + (i64)g * 32);
 #pragma unroll
-    for (int j = 1; j < 8; ++j) w[j] = 0u;
-  } else {
-    const int nw = (t1 - t) * (TILE_W / 32);
+      for (int j = 0; j < 8; ++j) {
+        const uint4 v = q[j];
+        wv[4 * j] = v.x;
+        wv[4 * j + 1] = v.y;
+        wv[4 * j + 2] = v.z;
+        wv[4 * j + 3] = v.w;
+      }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int wi = l * 8 + j;
-      w[j] = wi < nw ? src[wi] : 0u;
+      for (int j = 0; j < 32; ++j) {
+        mask |= (wv[j] != 0u ? 1u : 0u) << j;
+        ent += __popc(wv[j]);
+      }
     }
+    const u64 mine = ((u64)__popc(mask) << 32) | (u32)ent;
+    u64 tot;
+    const u64 inc = block_scan_incl<WG>(mine, scr, &tot);
+    const u64 base = carry + inc - mine;
+    if (g < ngroups) {
+      int slot = (int)(base >> 32);
+      u32 epre = (u32)base;
+      s_gt[g] = ((u64)(u32)slot << 32) | mask;
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        if (wv[j] != 0u) {
+          s_cwd[slot++] = ((u64)epre << 47) | ((u64)(u32)(g * 32 + j) << 32) | wv[j];
+          epre += __popc(wv[j]);
+        }
+      }
+    }
+    carry += tot;
+    __syncthreads();  // scr is reused by the next round's scan
   }
+  return (int)(carry >> 32);
 }
 
 // Writes a compressed step (accumulators acc[0..n) in rank order) at
-// out..out+n-1 of C with coalesced stores: the values first; then every
-// thread turns the accumulators of its own eight staged words (read with
-// independent loads: no chain of dependent LDS reads, which stall behind the
-// other workgroup's LDS traffic) into column offsets; these are written and
-// the array re-zeroed.  Three barriers.
+// out..out+n-1 of C with coalesced stores: the values first; then the
+// threads turn the accumulators of the staged words (slot l, l + WG, ...)
+// into column offsets in place; these are written and the array re-zeroed.
+// Three barriers.
 template <int WG>
-__device__ __forceinline__ void emit_ranked(const u64* cw, int n, i64 out, i64 cbase,
-                                            float* acc, i32* __restrict__ ccol,
+__device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n, i64 out,
+                                            i64 cbase, float* acc, i32* __restrict__ ccol,
                                             float* __restrict__ cval) {
   const int l = threadIdx.x;
   for (int q = l; q < n; q += WG) cval[out + q] = acc[q];
   __syncthreads();
   i32* colx = reinterpret_cast<i32*>(acc);
-  u64 mw[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) mw[j] = cw[l * 8 + j];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    u32 bits = (u32)mw[j];
-    int q = (int)(mw[j] >> 32);
-    const int x0 = (l * 8 + j) * 32;
+  for (int slot = l; slot < nwords; slot += WG) {
+    const u64 e = s_cwd[slot];
+    u32 bits = (u32)e;
+    int q = (int)(e >> 47);
+    const int x0 = (int)((e >> 32) & 0x7fff) * 32;
     while (bits) {
       colx[q++] = x0 + (__ffs(bits) - 1);
       bits &= bits - 1;
@@ -1175,16 +1215,18 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
     float* __restrict__ cval, i32* __restrict__ next) {
   constexpr int WG = HWG;
-  static_assert(TILE_W / 32 == WG && SEG_WORDS / WG == 8, "eight bitmap words per thread");
+  static_assert(TILE_W / 32 == WG, "one word of a dense tile per thread");
   __shared__ __align__(16) unsigned char s_raw[HSEG_LDS];
   __shared__ OffT s_off[WG];
   __shared__ float s_av[WG];
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
+  __shared__ u64 s_scr64[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   float* s_acc = reinterpret_cast<float*>(s_raw);
-  u64* s_cw = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4);
+  u64* s_cwd = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4);
+  u64* s_gt = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4 + SEG_MAXW * 8);
   const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_scr};
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
@@ -1232,65 +1274,51 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     int t = next_nonempty(s_tc, tc, nt, 0);
     int t1 = t, n = 0;
     bool dense = false;
-    u32 wv[8];
     if (t < nt) {
       seg_step(s_tc, tc, nt, t, t1, n, dense);
-      seg_words(bits, t, t1, dense, wv);
       if (inreg) {
 #pragma unroll
         for (int c = 0; c < HC; ++c) nxt[c] = ok[c] ? sp[c][t1] : 0;
       }
     }
     while (t < nt) {
-      // The next step: its extent, range ends and bitmap words (in flight
-      // while this step runs).
+      // The next step: its extent, and its range ends for the register
+      // entries (in flight while this step runs).
       const int tn = next_nonempty(s_tc, tc, nt, t1);
       int tn1 = tn, nn = 0;
       bool dn = false;
       i32 pre[HC];
-      u32 nwv[8];
 #pragma unroll
       for (int c = 0; c < HC; ++c) pre[c] = 0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) nwv[j] = 0u;
       if (tn < nt) {
         seg_step(s_tc, tc, nt, tn, tn1, nn, dn);
-        seg_words(bits, tn, tn1, dn, nwv);
         if (inreg) {
 #pragma unroll
           for (int c = 0; c < HC; ++c) pre[c] = ok[c] ? sp[c][tn1] : 0;
         }
       }
       const i64 cbase = (i64)t * TILE_W;
-      int wpfx = 0;  // compressed: prefix of this thread's first word
+      u32 dword = 0;  // dense: this thread's word of the tile
+      int nwords = 0;  // compressed: staged words
       if (dense) {
+        dword = bits[(i64)t * (TILE_W / 32) + l];
         if (upper_dirty) {
-          __syncthreads();  // the previous step's output has read its words
           for (int d = SEG_CAP + l; d < TILE_W; d += WG) s_acc[d] = 0.f;
           upper_dirty = false;
           __syncthreads();
         }
       } else {
-        int cnt = 0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cnt += __popc(wv[j]);
-        int tot;
-        const int inc = block_scan_incl<WG>(cnt, s_scr, &tot);
-        wpfx = inc - cnt;
-        int run = wpfx;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          s_cw[l * 8 + j] = ((u64)(u32)run << 32) | wv[j];
-          run += __popc(wv[j]);
-        }
+        nwords = stage_words<WG>(bits + (i64)t * (TILE_W / 32), (t1 - t) * (TILE_W / 32) / 32,
+                                 s_cwd, s_gt, s_scr64);
         upper_dirty = true;
-        __syncthreads();
       }
       auto add_dense = [&](i32 c, float p) { lds_add_f32(&s_acc[(int)(c - cbase)], p); };
       auto add_rank = [&](i32 c, float p) {
         const int x = (int)(c - cbase);
-        const u64 wd = s_cw[x >> 5];  // prefix in the high half, bits in the low half
-        const int rank = (int)(wd >> 32) + __popc((u32)wd & ((1u << (x & 31)) - 1u));
+        const u64 gg = s_gt[x >> 10];  // first slot << 32 | nonzero-word mask
+        const int slot = (int)(gg >> 32) + __popc((u32)gg & ((1u << ((x >> 5) & 31)) - 1u));
+        const u64 e = s_cwd[slot];  // entry prefix << 47 | word index << 32 | bits
+        const int rank = (int)(e >> 47) + __popc((u32)e & ((1u << (x & 31)) - 1u));
         lds_add_f32(&s_acc[rank], p);
       };
       if (inreg) {
@@ -1341,12 +1369,11 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
       // Output.  Dense tile: thread l writes the entries of its own word
       // (words of a dense tile carry similar counts) at the word's prefix.
       if (dense) {
-        const u32 b = wv[0];
-        const int pcb = (int)__popc(b);
+        const int pcb = (int)__popc(dword);
         int tot;
         const int inc = block_scan_incl<WG>(pcb, s_scr, &tot);
         i64 q = out + (inc - pcb);
-        u32 r = b;
+        u32 r = dword;
         while (r) {
           const int x = l * 32 + (__ffs(r) - 1);
           r &= r - 1;
@@ -1357,15 +1384,13 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         }
         __syncthreads();  // s_scr is reused by the next scan
       } else {
-        emit_ranked<WG>(s_cw, n, out, cbase, s_acc, ccol, cval);
+        emit_ranked<WG>(s_cwd, nwords, n, out, cbase, s_acc, ccol, cval);
       }
       out += n;
       t = tn;
       t1 = tn1;
       n = nn;
       dense = dn;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) wv[j] = nwv[j];
     }
     __syncthreads();  // s_tc is reloaded for the next row
   }
