@@ -50,6 +50,37 @@ __global__ __launch_bounds__(256) void k_fill_i32(i32* __restrict__ p, i64 n, i3
   for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256) p[i] = v;
 }
 
+// B's entries as (column, value bits) pairs, so that a numeric kernel
+// gathers a product's operands with one 8-byte load instead of two 4-byte
+// loads (half the vector-memory instructions and address work per product).
+__global__ __launch_bounds__(256) void k_pair_entries(i64 n, const i32* __restrict__ col,
+                                                      const float* __restrict__ val,
+                                                      uint2* __restrict__ cv) {
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += (i64)gridDim.x * 256)
+    cv[i] = make_uint2((u32)col[i], __float_as_uint(val[i]));
+}
+
+// Where a numeric kernel gathers B's entries from: separate column and value
+// arrays, or (column, value) pairs.
+struct SplitEntries {
+  const i32* __restrict__ col;
+  const float* __restrict__ val;
+  template <class OffT>
+  __device__ __forceinline__ void load(OffT i, i32& c, float& v) const {
+    c = col[i];
+    v = val[i];
+  }
+};
+struct PairedEntries {
+  const uint2* __restrict__ cv;
+  template <class OffT>
+  __device__ __forceinline__ void load(OffT i, i32& c, float& v) const {
+    const uint2 x = cv[i];
+    c = (i32)x.x;
+    v = __uint_as_float(x.y);
+  }
+};
+
 // Zeroes the row-slot counter of a one-row-per-workgroup kernel.
 void zero_slot(i32* next, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, next, (i64)1, 0);
@@ -229,11 +260,9 @@ struct ChunkLds {
 
 // f(col, prod) is called for every product: col = B column, prod = A*B value
 // (0 when VAL is false).  Returns the chunk's product count (uniform).
-template <int WG, int ROWS, bool VAL, class OffT, class F>
+template <int WG, int ROWS, bool VAL, class OffT, class Src, class F>
 __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int nvalid,
-                                              const i32* __restrict__ bcol,
-                                              const float* __restrict__ bval,
-                                              const ChunkLds<OffT>& s, F&& f) {
+                                              const Src& src, const ChunkLds<OffT>& s, F&& f) {
   static_assert(WG % WAVE == 0 && WG <= 65535, "marks are u16 entry indices + 1");
   constexpr int NW = WG / WAVE;
   constexpr int SB = ROWS * WAVE;  // products per sub-batch
@@ -296,8 +325,7 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         const OffT li = ok[r] ? idx[r] : (OffT)0;
-        c[r] = bcol[li];
-        if constexpr (VAL) v[r] = bval[li];
+        src.load(li, c[r], v[r]);
       }
 #pragma unroll
       for (int r = 0; r < ROWS; ++r)
@@ -847,7 +875,8 @@ __device__ __forceinline__ int tile_chunk(int len, i64 start, float av, i64 cbas
                                           const i32* __restrict__ bcol,
                                           const float* __restrict__ bval, float* s_vals,
                                           u32* s_bits, const ChunkLds<i64>& cl, int nvalid) {
-  return chunk_products<WG, 4, true>(len, start, av, nvalid, bcol, bval, cl, [&](i32 c, float p) {
+  return chunk_products<WG, 4, true>(len, start, av, nvalid, SplitEntries{bcol, bval}, cl,
+                                     [&](i32 c, float p) {
     const int x = (int)(c - cbase);
     lds_add_f32(&s_vals[x], p);
     lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
@@ -1091,33 +1120,53 @@ constexpr int HSEG_LDS = TILE_W * 4;
 static_assert(SEG_CAP * 4 + SEG_MAXW * 8 + SEG_GROUPS * 8 <= HSEG_LDS, "accumulator region");
 static_assert(SEG_MAXT * TILE_W <= (1 << 20) && SEG_CAP < (1 << 13), "staged word fields");
 
-// Step [t, t1) of a row from tile t (whose count c0 > 0): n entries, dense
-// when the tile alone exceeds SEG_CAP.  Trailing empty tiles are not kept.
-__device__ __forceinline__ void seg_step(const i32* s_tc, const i32* tc, int nt, int t, int& t1,
+// Plans the step that starts at the first nonempty tile at or after t (t is
+// moved there; t = nt when none is left).  Lane j of every wave looks at tile
+// t + j: a wave scan of the tile counts and of min(count, words per tile), a
+// bound on the nonzero words, and a ballot find the longest window within
+// SEG_MAXT tiles, SEG_CAP entries and SEG_MAXW words; trailing empty tiles
+// are dropped.  A tile of more than SEG_CAP entries is a dense step alone.
+// The results are uniform; every lane of the wave must be active.
+__device__ __forceinline__ void seg_plan(const i32* s_tc, const i32* tc, int nt, int& t, int& t1,
                                          int& n, bool& dense) {
-  const int c0 = t < TC_LDS ? s_tc[t] : tc[t];
-  t1 = t + 1;
-  n = c0;
-  dense = c0 > SEG_CAP;
-  if (!dense) {
-    int words = min(c0, TILE_W / 32);  // a bound on the nonzero words
-    int tend = t1;
-    while (t1 < nt && t1 - t < SEG_MAXT) {
-      const int ct = t1 < TC_LDS ? s_tc[t1] : tc[t1];
-      const int wt = min(ct, TILE_W / 32);
-      if (ct > SEG_CAP || n + ct > SEG_CAP || words + wt > SEG_MAXW) break;
-      n += ct;
-      words += wt;
-      ++t1;
-      if (ct > 0) tend = t1;
+  static_assert(SEG_MAXT == WAVE, "one tile per lane");
+  const int lane = threadIdx.x & (WAVE - 1);
+  int c;
+  for (;; t += WAVE) {
+    if (t >= nt) {
+      t = t1 = nt;
+      n = 0;
+      dense = false;
+      return;
     }
-    t1 = tend;
+    const int j = t + lane;
+    c = j < nt ? (j < TC_LDS ? s_tc[j] : tc[j]) : 0;
+    const u64 ne = __ballot(c > 0);
+    if (ne) {
+      const int f = __ffsll((long long)ne) - 1;
+      if (f > 0) {
+        t += f;
+        const int jj = t + lane;
+        c = jj < nt ? (jj < TC_LDS ? s_tc[jj] : tc[jj]) : 0;
+      }
+      break;
+    }
   }
-}
-
-__device__ __forceinline__ int next_nonempty(const i32* s_tc, const i32* tc, int nt, int t) {
-  while (t < nt && (t < TC_LDS ? s_tc[t] : tc[t]) == 0) ++t;
-  return t;
+  const int c0 = __builtin_amdgcn_readfirstlane(c);
+  dense = c0 > SEG_CAP;
+  if (dense) {
+    t1 = t + 1;
+    n = c0;
+    return;
+  }
+  const int sc = seg_scan_incl<WAVE>(c);
+  const int sw = seg_scan_incl<WAVE>(min(c, TILE_W / 32));
+  const u64 bad = __ballot(t + lane >= nt || sc > SEG_CAP || sw > SEG_MAXW);
+  const u64 keep = bad ? (1ull << (__ffsll((long long)bad) - 1)) - 1 : ~0ull;  // tile 0 fits
+  const u64 ne = __ballot(c > 0) & keep;
+  const int last = 63 - __clzll((long long)ne);
+  t1 = t + last + 1;
+  n = __shfl(sc, last);
 }
 
 // Stages the nonzero words of compressed step [t, t1) (see the LDS map):
@@ -1210,8 +1259,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
     const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const float* __restrict__ aval,
-    const i64* __restrict__ brp, const i32* __restrict__ bcol,
-    const float* __restrict__ bval, const i64* __restrict__ bbase,
+    const i64* __restrict__ brp, const uint2* __restrict__ bcv, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
     float* __restrict__ cval, i32* __restrict__ next) {
   constexpr int WG = HWG;
@@ -1228,6 +1276,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
   u64* s_cwd = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4);
   u64* s_gt = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4 + SEG_MAXW * 8);
   const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_scr};
+  const PairedEntries src{bcv};
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   const int l = threadIdx.x;
@@ -1271,11 +1320,10 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
       }
     }
     __syncthreads();  // s_tc
-    int t = next_nonempty(s_tc, tc, nt, 0);
-    int t1 = t, n = 0;
-    bool dense = false;
+    int t = 0, t1, n;
+    bool dense;
+    seg_plan(s_tc, tc, nt, t, t1, n, dense);
     if (t < nt) {
-      seg_step(s_tc, tc, nt, t, t1, n, dense);
       if (inreg) {
 #pragma unroll
         for (int c = 0; c < HC; ++c) nxt[c] = ok[c] ? sp[c][t1] : 0;
@@ -1284,14 +1332,13 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     while (t < nt) {
       // The next step: its extent, and its range ends for the register
       // entries (in flight while this step runs).
-      const int tn = next_nonempty(s_tc, tc, nt, t1);
-      int tn1 = tn, nn = 0;
-      bool dn = false;
+      int tn = t1, tn1, nn;
+      bool dn;
+      seg_plan(s_tc, tc, nt, tn, tn1, nn, dn);
       i32 pre[HC];
 #pragma unroll
       for (int c = 0; c < HC; ++c) pre[c] = 0;
       if (tn < nt) {
-        seg_step(s_tc, tc, nt, tn, tn1, nn, dn);
         if (inreg) {
 #pragma unroll
           for (int c = 0; c < HC; ++c) pre[c] = ok[c] ? sp[c][tn1] : 0;
@@ -1336,9 +1383,9 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
             }
           const int nvalid = (int)min((i64)WG, nA - (i64)c * WG);
           if (dense)
-            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_dense);
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_dense);
           else
-            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_rank);
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_rank);
         }
 #pragma unroll
         for (int c = 0; c < HC; ++c) {
@@ -1361,9 +1408,9 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
           }
           const int nvalid = (int)min((i64)WG, a1 - e0);
           if (dense)
-            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_dense);
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_dense);
           else
-            chunk_products<WG, ROWS, true>(len, start, a, nvalid, bcol, bval, cl, add_rank);
+            chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_rank);
         }
       }
       // Output.  Dense tile: thread l writes the entries of its own word
@@ -1645,6 +1692,14 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     DBuf<i32> next(1);
     zero_slot(next.p, s);
+    DBuf<uint2> pairs;
+    if (heavy.rowbits.p && B.nnz > 0) {
+      pairs.alloc((size_t)B.nnz);
+      ProfScope ps("pair_entries", s);
+      const unsigned pg = (unsigned)std::min<i64>((B.nnz + 255) / 256, (i64)device().cus * 16);
+      hipLaunchKernelGGL(k_pair_entries, dim3(pg), dim3(256), 0, s, B.nnz, B.col, B.val, pairs.p);
+      RT_HIP(hipGetLastError());
+    }
     {
       ProfScope ps("num_heavy", s);
       const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
@@ -1653,11 +1708,11 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
       if (heavy.rowbits.p && B.nnz < ((i64)1 << 31))
         hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 2, 8>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
-                           A.val, B.rp, B.col, B.val, bbase, slab, C.rp, C.col, C.val, next.p);
+                           A.val, B.rp, pairs.p, bbase, slab, C.rp, C.col, C.val, next.p);
       else if (heavy.rowbits.p)
         hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 4, 4>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
-                           A.val, B.rp, B.col, B.val, bbase, slab, C.rp, C.col, C.val, next.p);
+                           A.val, B.rp, pairs.p, bbase, slab, C.rp, C.col, C.val, next.p);
       else
         hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, key, tu.num[4], A.rp, A.col, A.val, B.rp,

@@ -110,8 +110,9 @@ def test_long_rows_uniform(n, k, seed):
 
 
 def test_long_rows_dense_and_compressed_steps():
-    """Rows of ~39K entries over 40K columns: the first 32768-column tile holds
-    more than 16384 entries (dense step), the rest a compressed step."""
+    """Rows of ~39K entries over 40K columns: the two full 16384-column tiles
+    hold more than 7168 entries each (dense steps), the last one is a
+    compressed step."""
     A = rt.gen_fixed_normal(200, 40000, 400, 41)
     B = rt.gen_uniform_k(40000, 400, 42)
     C, _ = run(A, B)
@@ -120,8 +121,8 @@ def test_long_rows_dense_and_compressed_steps():
 
 
 def test_long_rows_sparse_over_many_tiles():
-    """~5K entries spread over 31 tiles: compressed steps cut by the 8-tile
-    span limit rather than by their entry count."""
+    """~5K entries spread over 62 tiles: compressed steps cut by the bound on
+    staged bitmap words (3584) rather than by their entry count."""
     A = rt.gen_fixed_normal(100, 5000, 250, 43)
     B = rt.gen_fixed_normal(5000, 1_000_000, 20, 44)
     C, info = run(A, B)
@@ -137,10 +138,20 @@ def test_mixed_row_lengths_rmat():
 
 
 def test_wide_output_multiple_bitmap_windows():
-    """B with 1.5M columns: the symbolic bitmap takes two 2^20-column windows."""
+    """B with 1.5M columns: the symbolic bitmap takes three 2^19-column windows."""
     A = rt.gen_fixed_normal(64, 3000, 300, 6)
     B = rt.gen_fixed_normal(3000, 1_500_000, 40, 7)
     C, _ = run(A, B)
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
+def test_long_rows_window_span_limit():
+    """~4.8K entries over 92 tiles: a compressed step ends at its 64-tile span
+    before its entry or word bounds."""
+    A = rt.gen_fixed_normal(64, 3000, 120, 45)
+    B = rt.gen_fixed_normal(3000, 1_500_000, 40, 46)
+    C, _ = run(A, B)
+    assert np.diff(C.row_ptr).min() > 4096  # every row takes the long-row kernels
     check_parity(C, oracle.spgemm_f64(A, B))
 
 
