@@ -728,9 +728,10 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
 // row, two resident per CU, one bit per output column over a window of
 // SYM_WIN_TILES tiles (2^19 columns, 64 KB of LDS); wider matrices take
 // several windows, and the slab pointers of B restrict every window to its
-// own products, so each product is gathered once.  Records the nnz of every
-// tile of the row (the long-row numeric kernels' step plan and output
-// offsets) and, when rowbits is set, the row's bitmap.
+// own products, so each product is gathered once.  Records for every tile of
+// the row its entry count and its count of nonzero bitmap words, packed as
+// entries | words << 16 (the long-row numeric kernels' step plan and output
+// offsets), and, when rowbits is set, the row's bitmap.
 constexpr int SYM_WIN_TILES = 32;
 constexpr int SYM_WIN = SYM_WIN_TILES * TILE_W;
 
@@ -813,12 +814,15 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
       }
       i64 wsum = 0;
       for (int t = w; t < ntw; t += WG / WAVE) {
-        int c = 0;
+        int c = 0;  // entries | nonzero words << 16 (entries <= 16384)
         const int dend = min((t + 1) * (TILE_W / 32), nd);
-        for (int d = t * (TILE_W / 32) + lane; d < dend; d += WAVE) c += __popc(s_bits[d]);
+        for (int d = t * (TILE_W / 32) + lane; d < dend; d += WAVE) {
+          const u32 x = s_bits[d];
+          c += __popc(x) + (x != 0u ? 0x10000 : 0);
+        }
         c = seg_reduce_sum<WAVE>(c);
         if (lane == 0) tilecnt[it * nt + tw0 + t] = c;
-        wsum += c;
+        wsum += c & 0xffff;
       }
       total += block_reduce_sum<WG>(lane == 0 ? wsum : (i64)0, s_scr64);
     }
@@ -828,7 +832,7 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
 }
 
 // Per B row, the offset (relative to the row start) of the first entry in
-// each 32768-column tile: slab[k*(nt+1)+t], t = 0..nt.
+// each TILE_W-column tile: slab[k*(nt+1)+t], t = 0..nt.
 __global__ __launch_bounds__(256) void k_slab_ptrs(i64 brows, i32 nt, const i64* __restrict__ brp,
                                                    const i32* __restrict__ bcol,
                                                    i32* __restrict__ slab) {
@@ -1088,17 +1092,17 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
 // ---------------------------------------------------------------------------
 // Segmented long-row numeric kernel (exact method, row bitmaps available).
 //
-// The symbolic pass stored each long row's column set as a bitmap.  With it
-// the row is processed in few steps.  A "compressed" step covers up to
-// SEG_MAXT consecutive tiles (2^20 columns) holding at most SEG_CAP entries
-// in at most SEG_MAXW nonzero bitmap words: only the window's nonzero words
-// are staged in LDS (their entry prefix, word index and bits), with a table
-// of 32-word groups (the slot of the group's first nonzero word and a mask
-// of its nonzero words).  A product's rank is then two LDS reads away
-// (group -> word), so the accumulators sit in column order.  A tile holding
-// more than SEG_CAP entries is one "dense" step: 16384 LDS accumulators
-// indexed by column.  Output needs no search and no sort.  Steps end only on
-// entry or word capacity, so a sparse long row takes one or two steps.
+// The symbolic pass stored each long row's column set as a bitmap, and per
+// tile its entry count and nonzero-word count.  With them the row is
+// processed in few steps.  A "compressed" step covers up to SEG_MAXT
+// consecutive tiles (2^20 columns) whose accumulators, staged words and
+// group table fit the 64 KB accumulator region: only the window's nonzero
+// bitmap words are staged in LDS (entry prefix, word index, bits), with a
+// table of its 32-word groups (the slot of the group's first nonzero word
+// and a mask of its nonzero words).  A product's rank is two LDS reads away
+// (group -> word), so the accumulators sit in column order.  A first tile
+// holding more than SEG_DENSE entries is one "dense" step: 16384 LDS
+// accumulators indexed by column.  Output needs no search and no sort.
 //
 // Rows of up to HC*WG entries keep their entries in registers for the whole
 // row: B row start, slab row, A value, and a cursor into the B row that
@@ -1107,78 +1111,96 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
 // between hold no product of the row).  The next step's range ends are
 // loaded while the current step runs.  Longer rows re-read their entries per
 // step and chunk.
-constexpr int SEG_MAXT = 64;                                 // tiles per compressed window
-constexpr int SEG_CAP = 7168;                                // entries per compressed step
-constexpr int SEG_MAXW = 3584;                               // nonzero words per compressed step
-constexpr int SEG_GROUPS = SEG_MAXT * (TILE_W / 32) / 32;    // 32-word groups per window
-// LDS map of the accumulator region (bytes): dense steps use it whole as
-// 16384 accumulators; compressed steps use [0, 28K) as SEG_CAP accumulators,
-// [28K, 56K) as SEG_MAXW staged words (u64: entry prefix << 47 | word index
-// << 32 | bits) and [56K, 64K) as the group table (u64: first slot << 32 |
-// nonzero-word mask).
+constexpr int SEG_MAXT = 64;      // tiles per compressed step: one per lane of a planning wave
+constexpr int SEG_DENSE = 8192;   // a first tile with more entries is a dense step
 constexpr int HSEG_LDS = TILE_W * 4;
-static_assert(SEG_CAP * 4 + SEG_MAXW * 8 + SEG_GROUPS * 8 <= HSEG_LDS, "accumulator region");
-static_assert(SEG_MAXT * TILE_W <= (1 << 20) && SEG_CAP < (1 << 13), "staged word fields");
+constexpr int SEG_BUDGET = HSEG_LDS - 8;             // compressed-step bytes (alignment slack)
+constexpr int SEG_GPT = TILE_W / 32 / 32;            // 32-word groups per tile
+constexpr int SEG_TILE_GT = SEG_GPT * 8;             // group-table bytes per tile
+// Compressed-step layout of the accumulator region (bytes), for n entries in
+// w nonzero words over 16 groups per tile: accumulators [0, 4n); staged
+// words from OW = 4n rounded up to 8 (u64: entry prefix << 47 | word index
+// << 32 | bits); the group table from OW + 8w (u64: first slot << 32 |
+// nonzero-word mask).  A step's tiles cost 4c + 8w + 128 bytes each.
+static_assert(SEG_MAXT * TILE_W / 32 <= (1 << 15) && HSEG_LDS / 4 < (1 << 17),
+              "staged word fields");
 
-// Plans the step that starts at the first nonempty tile at or after t (t is
-// moved there; t = nt when none is left).  Lane j of every wave looks at tile
-// t + j: a wave scan of the tile counts and of min(count, words per tile), a
-// bound on the nonzero words, and a ballot find the longest window within
-// SEG_MAXT tiles, SEG_CAP entries and SEG_MAXW words; trailing empty tiles
-// are dropped.  A tile of more than SEG_CAP entries is a dense step alone.
-// The results are uniform; every lane of the wave must be active.
-__device__ __forceinline__ void seg_plan(const i32* s_tc, const i32* tc, int nt, int& t, int& t1,
-                                         int& n, bool& dense) {
+// Tile counts as the symbolic pass packs them: entries | nonzero words << 16.
+__device__ __forceinline__ int tile_entries(int v) { return v & 0xffff; }
+__device__ __forceinline__ int tile_words(int v) { return v >> 16; }
+
+struct SegStep {
+  int t, t1;   // tiles [t, t1)
+  int n, w;    // entries; nonzero bitmap words
+  bool dense;
+};
+
+__device__ __forceinline__ int tile_count(const i32* s_tc, const i32* tc, int nt, int j) {
+  return j < nt ? (j < TC_LDS ? s_tc[j] : tc[j]) : 0;
+}
+
+// Plans the step that starts at the first nonempty tile at or after t
+// (t = nt when none is left).  Lane j of every wave looks at tile t + j: wave
+// scans of the entry and word counts and a ballot find the longest window
+// within SEG_MAXT tiles and SEG_BUDGET bytes; trailing empty tiles are
+// dropped.  The results are uniform; every lane of the wave must be active.
+__device__ __forceinline__ SegStep seg_plan(const i32* s_tc, const i32* tc, int nt, int t) {
   static_assert(SEG_MAXT == WAVE, "one tile per lane");
   const int lane = threadIdx.x & (WAVE - 1);
-  int c;
+  SegStep p{nt, nt, 0, 0, false};
+  int v;
   for (;; t += WAVE) {
-    if (t >= nt) {
-      t = t1 = nt;
-      n = 0;
-      dense = false;
-      return;
-    }
-    const int j = t + lane;
-    c = j < nt ? (j < TC_LDS ? s_tc[j] : tc[j]) : 0;
-    const u64 ne = __ballot(c > 0);
+    if (t >= nt) return p;
+    v = tile_count(s_tc, tc, nt, t + lane);
+    const u64 ne = __ballot(v != 0);
     if (ne) {
       const int f = __ffsll((long long)ne) - 1;
       if (f > 0) {
         t += f;
-        const int jj = t + lane;
-        c = jj < nt ? (jj < TC_LDS ? s_tc[jj] : tc[jj]) : 0;
+        v = tile_count(s_tc, tc, nt, t + lane);
       }
       break;
     }
   }
-  const int c0 = __builtin_amdgcn_readfirstlane(c);
-  dense = c0 > SEG_CAP;
-  if (dense) {
-    t1 = t + 1;
-    n = c0;
-    return;
+  p.t = t;
+  const int c = tile_entries(v), wd = tile_words(v);
+  const int c0 = __builtin_amdgcn_readfirstlane(c), w0 = __builtin_amdgcn_readfirstlane(wd);
+  p.dense = c0 > SEG_DENSE || 4 * c0 + 8 * w0 + SEG_TILE_GT > SEG_BUDGET;
+  if (p.dense) {
+    p.t1 = t + 1;
+    p.n = c0;
+    p.w = w0;
+    return p;
   }
   const int sc = seg_scan_incl<WAVE>(c);
-  const int sw = seg_scan_incl<WAVE>(min(c, TILE_W / 32));
-  const u64 bad = __ballot(t + lane >= nt || sc > SEG_CAP || sw > SEG_MAXW);
+  const int sw = seg_scan_incl<WAVE>(wd);
+  const int bytes = 4 * sc + 8 * sw + SEG_TILE_GT * (lane + 1);
+  const u64 bad = __ballot(t + lane >= nt || bytes > SEG_BUDGET);
   const u64 keep = bad ? (1ull << (__ffsll((long long)bad) - 1)) - 1 : ~0ull;  // tile 0 fits
   const u64 ne = __ballot(c > 0) & keep;
   const int last = 63 - __clzll((long long)ne);
-  t1 = t + last + 1;
-  n = __shfl(sc, last);
+  p.t1 = t + last + 1;
+  p.n = __builtin_amdgcn_readfirstlane(__shfl(sc, last));
+  p.w = __builtin_amdgcn_readfirstlane(__shfl(sw, last));
+  return p;
 }
 
-// Stages the nonzero words of compressed step [t, t1) (see the LDS map):
-// thread l takes 32-word group l (and l + WG), counts its nonzero words and
-// entries, one block scan per round of WG groups gives the group's first
-// slot and entry prefix, and the group's words are written to their slots.
-// Returns the number of staged words.  Ends with a barrier.
+// Stages the nonzero words of compressed step [t, t1) (layout above).
+// Thread l takes the window's 32-word group g = l (then l + WG): the 16
+// groups of a tile sit on 16 consecutive lanes, whose scan of (nonzero
+// words, entries), added to the tile's own prefixes (lane j of every wave
+// computes tile j's), gives each group's first slot and entry prefix.  Ends
+// with a barrier.
 template <int WG>
-__device__ __forceinline__ int stage_words(const u32* __restrict__ src, int ngroups, u64* s_cwd,
-                                           u64* s_gt, u64* scr) {
+__device__ __forceinline__ void stage_words(const u32* __restrict__ src, const i32* s_tc,
+                                            const i32* tc, int nt, int t, int t1, u64* s_cwd,
+                                            u64* s_gt) {
   const int l = threadIdx.x;
-  u64 carry = 0;  // (words << 32) | entries before this round
+  const int lane = l & (WAVE - 1);
+  const int v = lane < t1 - t ? tile_count(s_tc, tc, nt, t + lane) : 0;
+  const int c = tile_entries(v), wd = tile_words(v);
+  const int pc = seg_scan_incl<WAVE>(c) - c, pw = seg_scan_incl<WAVE>(wd) - wd;
+  const int ngroups = (t1 - t) * SEG_GPT;
   for (int g0 = 0; g0 < ngroups; g0 += WG) {
     const int g = g0 + l;
     u32 wv[32];
@@ -1188,11 +1210,11 @@ __device__ __forceinline__ int stage_words(const u32* __restrict__ src, int ngro
       const uint4* q = reinterpret_cast<const uint4*>(src + (i64)g * 32);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const uint4 v = q[j];
-        wv[4 * j] = v.x;
-        wv[4 * j + 1] = v.y;
-        wv[4 * j + 2] = v.z;
-        wv[4 * j + 3] = v.w;
+        const uint4 x = q[j];
+        wv[4 * j] = x.x;
+        wv[4 * j + 1] = x.y;
+        wv[4 * j + 2] = x.z;
+        wv[4 * j + 3] = x.w;
       }
 #pragma unroll
       for (int j = 0; j < 32; ++j) {
@@ -1201,12 +1223,12 @@ __device__ __forceinline__ int stage_words(const u32* __restrict__ src, int ngro
       }
     }
     const u64 mine = ((u64)__popc(mask) << 32) | (u32)ent;
-    u64 tot;
-    const u64 inc = block_scan_incl<WG>(mine, scr, &tot);
-    const u64 base = carry + inc - mine;
+    const u64 ex = seg_scan_incl<SEG_GPT>(mine) - mine;
+    const int tj = (g / SEG_GPT) & (WAVE - 1);  // the group's tile in the window (< 64)
+    const int tpw = __shfl(pw, tj), tpc = __shfl(pc, tj);
     if (g < ngroups) {
-      int slot = (int)(base >> 32);
-      u32 epre = (u32)base;
+      int slot = tpw + (int)(ex >> 32);
+      u32 epre = (u32)tpc + (u32)ex;
       s_gt[g] = ((u64)(u32)slot << 32) | mask;
 #pragma unroll
       for (int j = 0; j < 32; ++j) {
@@ -1216,10 +1238,8 @@ __device__ __forceinline__ int stage_words(const u32* __restrict__ src, int ngro
         }
       }
     }
-    carry += tot;
-    __syncthreads();  // scr is reused by the next round's scan
   }
-  return (int)(carry >> 32);
+  __syncthreads();
 }
 
 // Writes a compressed step (accumulators acc[0..n) in rank order) at
@@ -1270,11 +1290,8 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
-  __shared__ u64 s_scr64[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   float* s_acc = reinterpret_cast<float*>(s_raw);
-  u64* s_cwd = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4);
-  u64* s_gt = reinterpret_cast<u64*>(s_raw + SEG_CAP * 4 + SEG_MAXW * 8);
   const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_scr};
   const PairedEntries src{bcv};
   const i32 nlist = rs.count();
@@ -1283,7 +1300,8 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
   const i64 row_words = (i64)nt * (TILE_W / 32);
   for (int d = l; d < TILE_W; d += WG) s_acc[d] = 0.f;
   for (int d = l; d < ROWS * WG; d += WG) s_mark[d] = 0;
-  bool upper_dirty = false;  // s_acc[SEG_CAP, TILE_W) holds staged words
+  // s_acc[dz0, dz1) may hold staged words and group entries; the rest is 0.
+  int dz0 = 0, dz1 = 0;
   __syncthreads();
   __shared__ i32 s_slot;
   for (i64 h = next_slot(next, &s_slot); h < nlist; h = next_slot(next, &s_slot)) {
@@ -1320,44 +1338,47 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
       }
     }
     __syncthreads();  // s_tc
-    int t = 0, t1, n;
-    bool dense;
-    seg_plan(s_tc, tc, nt, t, t1, n, dense);
-    if (t < nt) {
+    SegStep P = seg_plan(s_tc, tc, nt, 0);
+    if (P.t < nt) {
       if (inreg) {
 #pragma unroll
-        for (int c = 0; c < HC; ++c) nxt[c] = ok[c] ? sp[c][t1] : 0;
+        for (int c = 0; c < HC; ++c) nxt[c] = ok[c] ? sp[c][P.t1] : 0;
       }
     }
-    while (t < nt) {
+    while (P.t < nt) {
       // The next step: its extent, and its range ends for the register
       // entries (in flight while this step runs).
-      int tn = t1, tn1, nn;
-      bool dn;
-      seg_plan(s_tc, tc, nt, tn, tn1, nn, dn);
+      const SegStep Q = seg_plan(s_tc, tc, nt, P.t1);
       i32 pre[HC];
 #pragma unroll
       for (int c = 0; c < HC; ++c) pre[c] = 0;
-      if (tn < nt) {
+      if (Q.t < nt) {
         if (inreg) {
 #pragma unroll
-          for (int c = 0; c < HC; ++c) pre[c] = ok[c] ? sp[c][tn1] : 0;
+          for (int c = 0; c < HC; ++c) pre[c] = ok[c] ? sp[c][Q.t1] : 0;
         }
       }
+      const int t = P.t, t1 = P.t1, n = P.n;
       const i64 cbase = (i64)t * TILE_W;
+      u64* s_cwd = reinterpret_cast<u64*>(s_raw + ((4 * n + 7) & ~7));
+      u64* s_gt = s_cwd + P.w;
       u32 dword = 0;  // dense: this thread's word of the tile
-      int nwords = 0;  // compressed: staged words
-      if (dense) {
+      if (P.dense) {
         dword = bits[(i64)t * (TILE_W / 32) + l];
-        if (upper_dirty) {
-          for (int d = SEG_CAP + l; d < TILE_W; d += WG) s_acc[d] = 0.f;
-          upper_dirty = false;
+        if (dz1 > dz0) {
+          for (int d = dz0 + l; d < dz1; d += WG) s_acc[d] = 0.f;
+          dz0 = dz1 = 0;
           __syncthreads();
         }
       } else {
-        nwords = stage_words<WG>(bits + (i64)t * (TILE_W / 32), (t1 - t) * (TILE_W / 32) / 32,
-                                 s_cwd, s_gt, s_scr64);
-        upper_dirty = true;
+        // Accumulators [0, n) start at zero: clear what staged data they
+        // overlap (ordered before the products by the staging barrier).
+        for (int d = dz0 + l; d < min(dz1, n); d += WG) s_acc[d] = 0.f;
+        stage_words<WG>(bits + (i64)t * (TILE_W / 32), s_tc, tc, nt, t, t1, s_cwd, s_gt);
+        const int end = (int)((reinterpret_cast<unsigned char*>(s_gt + (t1 - t) * SEG_GPT) -
+                               s_raw) / 4);
+        dz0 = n;  // the output re-zeroes [0, n)
+        dz1 = max(dz1, end);
       }
       auto add_dense = [&](i32 c, float p) { lds_add_f32(&s_acc[(int)(c - cbase)], p); };
       auto add_rank = [&](i32 c, float p) {
@@ -1382,7 +1403,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
               a = av[cc];
             }
           const int nvalid = (int)min((i64)WG, nA - (i64)c * WG);
-          if (dense)
+          if (P.dense)
             chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_dense);
           else
             chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_rank);
@@ -1407,7 +1428,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
             a = aval[e];
           }
           const int nvalid = (int)min((i64)WG, a1 - e0);
-          if (dense)
+          if (P.dense)
             chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_dense);
           else
             chunk_products<WG, ROWS, true>(len, start, a, nvalid, src, cl, add_rank);
@@ -1415,7 +1436,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
       }
       // Output.  Dense tile: thread l writes the entries of its own word
       // (words of a dense tile carry similar counts) at the word's prefix.
-      if (dense) {
+      if (P.dense) {
         const int pcb = (int)__popc(dword);
         int tot;
         const int inc = block_scan_incl<WG>(pcb, s_scr, &tot);
@@ -1431,13 +1452,10 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         }
         __syncthreads();  // s_scr is reused by the next scan
       } else {
-        emit_ranked<WG>(s_cwd, nwords, n, out, cbase, s_acc, ccol, cval);
+        emit_ranked<WG>(s_cwd, P.w, n, out, cbase, s_acc, ccol, cval);
       }
       out += n;
-      t = tn;
-      t1 = tn1;
-      n = nn;
-      dense = dn;
+      P = Q;
     }
     __syncthreads();  // s_tc is reloaded for the next row
   }

@@ -121,8 +121,9 @@ def test_long_rows_dense_and_compressed_steps():
 
 
 def test_long_rows_sparse_over_many_tiles():
-    """~5K entries spread over 62 tiles: compressed steps cut by the bound on
-    staged bitmap words (3584) rather than by their entry count."""
+    """~5K entries spread over 62 tiles: compressed steps cut by the LDS
+    budget (accumulators, staged bitmap words and group table) before the
+    64-tile span, each staged in two rounds of 32 tiles."""
     A = rt.gen_fixed_normal(100, 5000, 250, 43)
     B = rt.gen_fixed_normal(5000, 1_000_000, 20, 44)
     C, info = run(A, B)
