@@ -88,13 +88,16 @@ void zero_slot(i32* next, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Row analysis: P_i = sum over A(i,k) of nnz(B(k,:)).  16 lanes per row,
-// rows grid-stride; totals[0] += sum P_i, totals[1] = max P_i, with one
+// rows grid-stride; rows of more than RP_LONG entries are queued for a
+// workgroup each.  totals[0] += sum P_i, totals[1] = max P_i, with one
 // atomic of each kind per workgroup (same-address atomics serialise).
+constexpr i64 RP_LONG = 1024;  // longer A rows go to the workgroup-per-row kernel
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_row_products(
     i64 nrows, const i64* __restrict__ arp, const i32* __restrict__ acol,
     const i64* __restrict__ brp, const i64* __restrict__ bbase, i64* __restrict__ prod,
-    unsigned long long* __restrict__ totals) {
+    unsigned long long* __restrict__ totals, i32* __restrict__ longq) {
   constexpr int G = 16;
   __shared__ i64 s_scr[4];
   const int l = threadIdx.x & (G - 1);
@@ -102,16 +105,22 @@ __global__ __launch_bounds__(256) void k_row_products(
   for (i64 r0 = (i64)blockIdx.x * (256 / G); r0 < nrows; r0 += (i64)gridDim.x * (256 / G)) {
     const i64 row = r0 + threadIdx.x / G;
     i64 sum = 0;
-    if (row < nrows) {
+    bool mine = row < nrows;
+    if (mine) {
       const i64 a0 = arp[row], a1 = arp[row + 1];
-      const i64 bb = BATCH ? bbase[row] : 0;
-      for (i64 e = a0 + l; e < a1; e += G) {
-        const i64 k = (i64)acol[e] + bb;
-        sum += brp[k + 1] - brp[k];
+      if (a1 - a0 > RP_LONG) {
+        mine = false;  // queued for k_row_products_long
+        if (l == 0) longq[1 + atomicAdd(&longq[0], 1)] = (i32)row;
+      } else {
+        const i64 bb = BATCH ? bbase[row] : 0;
+        for (i64 e = a0 + l; e < a1; e += G) {
+          const i64 k = (i64)acol[e] + bb;
+          sum += brp[k + 1] - brp[k];
+        }
       }
     }
     sum = seg_reduce_sum<G>(sum);
-    if (row < nrows && l == 0) prod[row] = sum;
+    if (mine && l == 0) prod[row] = sum;
     if (l == 0) {
       tsum += sum;
       tmax = max(tmax, sum);
@@ -123,10 +132,64 @@ __global__ __launch_bounds__(256) void k_row_products(
   if (threadIdx.x == 0 && ws > 0) atomicAdd(&totals[0], (unsigned long long)ws);
 }
 
+// The rows of more than RP_LONG A entries (longq[0] of them, listed from
+// longq[1]): one workgroup per row, four independent entries per thread in
+// flight, so the longest rows no longer serialise on 16 lanes.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_row_products_long(
+    const i32* __restrict__ longq, const i64* __restrict__ arp, const i32* __restrict__ acol,
+    const i64* __restrict__ brp, const i64* __restrict__ bbase, i64* __restrict__ prod,
+    unsigned long long* __restrict__ totals) {
+  constexpr int U = 4;
+  __shared__ i64 s_scr[4];
+  const i32 n = longq[0];
+  for (i32 i = blockIdx.x; i < n; i += gridDim.x) {
+    const i32 row = longq[1 + i];
+    const i64 a0 = arp[row], a1 = arp[row + 1];
+    const i64 bb = BATCH ? bbase[row] : 0;
+    i64 sum = 0;
+    for (i64 e0 = a0 + threadIdx.x; e0 < a1; e0 += 256 * U) {
+      i64 k[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const i64 e = e0 + u * 256;
+        k[u] = e < a1 ? (i64)acol[e] + bb : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k[u] >= 0) sum += brp[k[u] + 1] - brp[k[u]];
+    }
+    sum = block_reduce_sum<256>(sum, s_scr);
+    if (threadIdx.x == 0) {
+      prod[row] = sum;
+      atomicAdd(&totals[0], (unsigned long long)sum);
+      atomicMax(&totals[1], (unsigned long long)sum);
+    }
+  }
+}
+
 // Grid of the row-analysis kernel: enough workgroups to fill the device.
 inline unsigned row_products_grid(i64 nrows) {
   const i64 need = (nrows * 16 + 255) / 256;
   return (unsigned)std::max<i64>(1, std::min<i64>(need, (i64)device().cus * 16));
+}
+
+// Entries of the long-row queue of the row analysis (count + rows).
+inline size_t row_products_queue(i64 annz) { return (size_t)(annz / RP_LONG + 2); }
+
+// Row analysis: prod[i] = P_i; totals (zeroed by the caller) += (sum, max).
+// longq holds row_products_queue(A.nnz) entries.  Kernels only (graph
+// capture records them).
+template <bool BATCH>
+void launch_row_products(i64 m, const i64* arp, const i32* acol, const i64* brp,
+                         const i64* bbase, i64* prod, unsigned long long* totals, i32* longq,
+                         hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, longq, (i64)1, 0);
+  hipLaunchKernelGGL(k_row_products<BATCH>, dim3(row_products_grid(m)), dim3(256), 0, s, m, arp,
+                     acol, brp, bbase, prod, totals, longq);
+  hipLaunchKernelGGL(k_row_products_long<BATCH>, dim3((unsigned)device().cus * 2), dim3(256), 0,
+                     s, longq, arp, acol, brp, bbase, prod, totals);
 }
 
 // ---------------------------------------------------------------------------
@@ -1808,13 +1871,13 @@ void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipSt
   RT_HIP(hipMemsetAsync(tot.p, 0, 2 * sizeof(unsigned long long), s));
   if (m > 0) {
     ProfScope ps("row_products", s);
-    const dim3 grid(row_products_grid(m));
+    DBuf<i32> longq(row_products_queue(A.nnz));
     if (bbase)
-      hipLaunchKernelGGL(k_row_products<true>, grid, dim3(256), 0, s, m, A.rp, A.col, B.rp, bbase,
-                         prod, tot.p);
+      launch_row_products<true>(m, A.rp, A.col, B.rp, bbase, prod, tot.p, longq.p, s);
     else
-      hipLaunchKernelGGL(k_row_products<false>, grid, dim3(256), 0, s, m, A.rp, A.col, B.rp,
-                         nullptr, prod, tot.p);
+      launch_row_products<false>(m, A.rp, A.col, B.rp, nullptr, prod, tot.p, longq.p, s);
+    RT_HIP(hipGetLastError());
+    RT_HIP(hipStreamSynchronize(s));  // longq is released at scope exit
   }
   RT_HIP(hipGetLastError());
   unsigned long long h[2];
@@ -2042,6 +2105,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
            "graph mode: tile-count scratch too large; use the direct launch path");
   DBuf<i64> prod(M), nnz(M);
   DBuf<unsigned long long> tot(2);
+  DBuf<i32> longq(row_products_queue(A.nnz));
   DevBins sb(m), nb(m);
   DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
   const i64 nbs = std::max<i64>(1, (m + SCAN_TILE - 1) / SCAN_TILE);
@@ -2057,9 +2121,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
     ms_a = run_captured(s, [&] {
       hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(tot.p),
                          (i64)2, (i64)0);
-      if (m > 0)
-        hipLaunchKernelGGL(k_row_products<BATCH>, dim3(row_products_grid(m)),
-                           dim3(256), 0, s, m, A.rp, A.col, B.rp, bbase, prod.p, tot.p);
+      launch_row_products<BATCH>(m, A.rp, A.col, B.rp, bbase, prod.p, tot.p, longq.p, s);
       hipLaunchKernelGGL(k_fill_i64, dim3(fill_grid(M)), dim3(256), 0, s, nnz.p, M, (i64)0);
       bin_rows_dev(prod.p, m, B.cols, tu.sym, sb, s);
       HashArgs a = base;
