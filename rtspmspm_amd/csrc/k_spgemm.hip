@@ -20,7 +20,6 @@ using namespace dev;
 
 constexpr int NBIN = 6;
 constexpr int TILE_W = 16384;                   // columns per dense numeric tile
-constexpr int UNR = 8;                          // products gathered per lane per step
 // Threads of the long-row numeric kernels: one presence word per thread of a
 // tile, and small enough LDS (< 80 KB) for two workgroups per CU.
 constexpr int HWG = TILE_W / 32;
@@ -78,6 +77,15 @@ struct PairedEntries {
     const uint2 x = cv[i];
     c = (i32)x.x;
     v = __uint_as_float(x.y);
+  }
+};
+
+// Column indices only (symbolic kernels): the value is left unset.
+struct ColumnEntries {
+  const i32* __restrict__ col;
+  template <class OffT>
+  __device__ __forceinline__ void load(OffT i, i32& c, float&) const {
+    c = col[i];
   }
 };
 
@@ -252,52 +260,7 @@ __device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo,
   return lo;
 }
 
-// Per-lane walk over a chunk's flat product range for workgroup-wide loops.
-// The chunk's products [0, tot) are cut into one contiguous slice per wave;
-// lane l of a wave takes slice products l, l+64, ..., so each memory
-// instruction of the wave still reads 64 consecutive products.  The lane
-// keeps its current A entry (start of the next entry, B offset, A value) in
-// registers and touches LDS only when it crosses into another entry.
-struct ProdCursor {
-  int p, end, lo, nx;
-  i64 off;  // B index of product p is off + p
-  float a;
-
-  template <int WG>
-  __device__ __forceinline__ void init(int tot, int nvalid, const i32* ex, const i64* bs,
-                                       const float* av) {
-    constexpr int NWV = WG / WAVE;
-    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
-    const int per = (tot + NWV - 1) / NWV;
-    const int beg = w * per;
-    end = min(tot, beg + per);
-    p = beg + lane;
-    lo = 0;
-    nx = 0;
-    off = 0;
-    a = 0.f;
-    if (p < end) refresh(nvalid, ex, bs, av);
-  }
-  __device__ __forceinline__ void refresh(int nvalid, const i32* ex, const i64* bs,
-                                          const float* av) {
-    lo = advance_entry(ex, lo, nvalid, p);
-    nx = lo + 1 < nvalid ? ex[lo + 1] : INT_MAX;
-    off = bs[lo] - ex[lo];
-    a = av ? av[lo] : 0.f;
-  }
-  // Next product of this lane: its B index and A value; false when done.
-  __device__ __forceinline__ bool next(int nvalid, const i32* ex, const i64* bs, const float* av,
-                                       i64& idx, float& aval) {
-    if (p >= end) return false;
-    if (p >= nx) refresh(nvalid, ex, bs, av);
-    idx = off + p;
-    aval = a;
-    p += WAVE;
-    return true;
-  }
-};
-
-// Product enumeration by owner marks, for the long-row numeric kernels.  A
+// Product enumeration by owner marks, for the long-row kernels.  A
 // chunk of up to WG entries A(i,k) -- entry l held by thread l as (run
 // length, first B index, A value) -- has products 0..tot-1, entry e owning
 // [ex[e], ex[e] + len[e]).  After one block scan the waves work
@@ -798,7 +761,7 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
 constexpr int SYM_WIN_TILES = 32;
 constexpr int SYM_WIN = SYM_WIN_TILES * TILE_W;
 
-template <bool BATCH>
+template <bool BATCH, class OffT, int ROWS>
 __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
     RowSet rs, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const i64* __restrict__ brp,
@@ -810,12 +773,16 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   __shared__ u32 s_bits[WW];
-  __shared__ i64 s_bs[WG];
-  __shared__ i32 s_ex[WG];
+  __shared__ OffT s_off[WG];
+  __shared__ i32 s_ex[WG + 1];
+  __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i64 s_scr64[WG / WAVE];
+  const ChunkLds<OffT> cl{s_off, nullptr, s_ex, s_mark, s_scr};
+  const ColumnEntries src{bcol};
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1), w = l / WAVE;
+  for (int d = l; d < ROWS * WG; d += WG) s_mark[d] = 0;
   __shared__ i32 s_slot;
   for (i64 it = next_slot(next, &s_slot); it < nlist; it = next_slot(next, &s_slot)) {
     const i32 row = list[it];
@@ -832,43 +799,19 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
       for (i64 e0 = a0; e0 < a1; e0 += WG) {
         const i64 e = e0 + l;
         int len = 0;
-        i64 bs = 0;
+        OffT start = 0;
         if (e < a1) {
           const i64 k = (i64)acol[e] + bb;
           const i32* sk = slab + k * (nt + 1);
           const i32 s0 = sk[tw0];
-          bs = brp[k] + s0;
+          start = (OffT)(brp[k] + s0);
           len = sk[tw1] - s0;
         }
-        int tot;
-        const int inc = block_scan_incl<WG>(len, s_scr, &tot);
-        s_ex[l] = inc - len;
-        s_bs[l] = bs;
-        __syncthreads();
         const int nvalid = (int)min((i64)WG, a1 - e0);
-        ProdCursor cur;
-        cur.init<WG>(tot, nvalid, s_ex, s_bs, nullptr);
-        while (true) {
-          i64 idx[UNR];
-          bool any = false;
-#pragma unroll
-          for (int u = 0; u < UNR; ++u) {
-            float unused;
-            if (cur.next(nvalid, s_ex, s_bs, nullptr, idx[u], unused)) any = true;
-            else idx[u] = -1;
-          }
-          if (!any) break;
-          i32 cc[UNR];
-#pragma unroll
-          for (int u = 0; u < UNR; ++u) cc[u] = bcol[idx[u] >= 0 ? idx[u] : 0];  // unconditional
-#pragma unroll
-          for (int u = 0; u < UNR; ++u)
-            if (idx[u] >= 0) {
-              const int c = (int)((i64)cc[u] - win);  // in [0, wcols) by the slab bounds
-              lds_or_u32(&s_bits[c >> 5], 1u << (c & 31));
-            }
-        }
-        __syncthreads();
+        chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl, [&](i32 c, float) {
+          const int x = (int)((i64)c - win);  // in [0, wcols) by the slab bounds
+          lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
+        });
       }
       const int ntw = tw1 - tw0;
       if (rowbits) {  // keep the row's column set (padded to whole tiles) for the numeric pass
@@ -1704,6 +1647,20 @@ void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const 
   launch_hash<512, 8192, 512, MODE, BATCH>(names[MODE][4], a, list + off[4], cnt[4], s);
 }
 
+// int32 B offsets (and eight products per lane in flight) when B's entries
+// are indexable by int32.
+template <bool BATCH>
+void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr& B,
+                       const i64* bbase, const i32* slab, i32 nt, i32* tilecnt, i64* nnz,
+                       u32* rowbits, i32* next, hipStream_t s) {
+  if (B.nnz < ((i64)1 << 31))
+    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8>), dim3(grid), dim3(HWG), 0, s, rs, A.rp,
+                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, next);
+  else
+    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i64, 4>), dim3(grid), dim3(HWG), 0, s, rs, A.rp,
+                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, next);
+}
+
 HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
   HashArgs a{};
   a.arp = A.rp;
@@ -1752,9 +1709,8 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     zero_slot(heavy.next.p, s);
     ProfScope ps("sym_bitmap", s);
     const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
-    hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(grid), dim3(HWG), 0, s, rs, A.rp, A.col, B.rp,
-                       B.col, bbase, heavy.slab.p, B.cols, heavy.nt, heavy.tilecnt.p, nnz,
-                       heavy.rowbits.p, heavy.next.p);
+    launch_sym_bitmap<BATCH>(grid, rs, A, B, bbase, heavy.slab.p, heavy.nt, heavy.tilecnt.p, nnz,
+                             heavy.rowbits.p, heavy.next.p, s);
     RT_HIP(hipGetLastError());
   }
 }
@@ -2133,9 +2089,8 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
                            s, B.rows, nt, B.rp, B.col, slab.p);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p, s);
-      hipLaunchKernelGGL(k_sym_bitmap<BATCH>, dim3(heavy_grid * 2), dim3(HWG), 0, s, heavy, A.rp,
-                         A.col, B.rp, B.col, bbase, slab.p, B.cols, nt, tilecnt.p, nnz.p, nullptr,
-                         next.p);
+      launch_sym_bitmap<BATCH>(heavy_grid * 2, heavy, A, B, bbase, slab.p, nt, tilecnt.p, nnz.p,
+                               nullptr, next.p, s);
       hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
       hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nbs, C.rp + m);
       if (m > 0)
