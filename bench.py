@@ -10,6 +10,10 @@ flops = 2P; value = 2P / t in GFLOP/s.  Algorithmic bytes of the whole
 product (A read once, C written once, CSR with int64 row pointers and
 4+4-byte entries): 8(m+1) + 8 nnz(A) + 8(m+1) + 8 nnz(C).
 
+Before timing, the library's autotuner picks the row-bin thresholds for this
+matrix class for each method timed (untimed, cached in a temporary tuning
+file); both methods then run with their tuned settings.
+
 Prints ONE JSON line on stdout (human notes go to stderr).
 """
 import argparse
@@ -28,6 +32,12 @@ HBM_PEAK_GBS = 8000.0
 TH = [16, 64, 256, 1024, 4096]  # default row-bin thresholds of the library
 
 
+def parse_thresholds(report):
+    """The numeric-pass thresholds from an autotuner report line."""
+    words = report.split("thresholds sym", 1)[1].split()
+    return [int(x) for x in words[6:11]]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -37,10 +47,12 @@ def row_products(A):
     return np.add.reduceat(np.append(lens[A.col], 0), A.row_ptr[:-1]) * (np.diff(A.row_ptr) > 0)
 
 
-def kernel_bytes(name, A, P, cnnz, ncols):
+def kernel_bytes(name, A, P, cnnz, ncols, TH=TH):
     """Algorithmic bytes of one launch of a named kernel: row pointers (16 B
     per row), A entries it must read (col 4 B, +val 4 B for numeric), C
-    entries it writes (8 B) or row counts (8 B)."""
+    entries it writes (8 B) or row counts (8 B).  TH: the numeric thresholds
+    the product ran with (the symbolic ones are equal in every tuning
+    candidate)."""
     anz = np.diff(A.row_ptr)
     U = np.minimum(P, ncols)
 
@@ -241,6 +253,8 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run with the library's default row-bin thresholds")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's device_copy kernel over BYTES (counter "
                          "calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -262,6 +276,10 @@ def main():
         run_workload(args)
         return
 
+    if not args.no_tune:
+        import tempfile
+        os.environ.setdefault("RTSPMSPM_TUNE_CACHE",
+                              os.path.join(tempfile.gettempdir(), f"rtspmspm_bench_{os.getpid()}.txt"))
     import rtspmspm_amd as rt
     dev = rt.device_info()
     log(f"device {dev['name']}, {dev['cus']} CUs, {dev['total_mem'] / 2**30:.0f} GiB")
@@ -272,18 +290,31 @@ def main():
     log(f"R-MAT s{args.scale}: n={A.shape[0]} nnz={A.nnz} products={ptot} "
         f"(gen {time.perf_counter() - t0:.1f} s)")
     dA = rt.to_device(A)
+    tuned = not args.no_tune
+    tuning = {}
+    th_used = TH
+    if tuned:
+        for method in dict.fromkeys((["plain"] if args.compare and args.method != "plain" else [])
+                                    + [args.method]):
+            if method == "ray":
+                continue
+            t1 = time.perf_counter()
+            tuning[method] = rt.autotune(dA, dA, mode=method, reps=2)
+            log(f"autotune {method} ({time.perf_counter() - t1:.1f} s): {tuning[method]}")
+        if args.method in tuning:
+            th_used = parse_thresholds(tuning[args.method])
 
     def timed(method):
         """Warm-up, then K timed steps; returns (wall ms/step, device ms list,
         last C's per-row nnz, last info)."""
         for _ in range(args.warmup):
-            C, info = rt.spmspm(dA, dA, mode=method)
+            C, info = rt.spmspm(dA, dA, mode=method, use_tuned=tuned)
             C.free()
         rt.synchronize()
         ms_dev = []
         t0 = time.perf_counter()
         for s in range(args.steps):
-            C, info = rt.spmspm(dA, dA, mode=method)
+            C, info = rt.spmspm(dA, dA, mode=method, use_tuned=tuned)
             ms_dev.append(info["ms"])
             if s + 1 < args.steps:
                 C.free()
@@ -318,14 +349,14 @@ def main():
     kernels = {}
     if not args.no_profile:
         rt.prof_reset()
-        C, _ = rt.spmspm(dA, dA, mode=args.method, profile=True)
+        C, _ = rt.spmspm(dA, dA, mode=args.method, profile=True, use_tuned=tuned)
         C.free()
         stats = rt.prof_stats()
         kernels = {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in stats.items()}
         dom = max(stats, key=lambda k: stats[k][1])
         launches, tot_ms = stats[dom]
         avg_s = tot_ms / launches / 1e3
-        kb = kernel_bytes(dom, A, P, cnnz_row, A.shape[1])
+        kb = kernel_bytes(dom, A, P, cnnz_row, A.shape[1], th_used)
         achieved = kb / avg_s / 1e9 if kb else None
         traffic = None
         pmc = os.path.join(ROOT, "profiles", f"{dom}_rmat{args.scale}_traffic.json")
@@ -356,7 +387,10 @@ def main():
                    "bytes_alg": bytes_alg,
                    "pipeline_gbs": round(bytes_alg / ms_per_step / 1e6, 2),
                    "pipeline_roofline_frac": round(bytes_alg / ms_per_step / 1e6 / HBM_PEAK_GBS, 5),
-                   "device_ms_median": round(float(np.median(ms_dev)), 3), **compare},
+                   "device_ms_median": round(float(np.median(ms_dev)), 3),
+                   "tuned": tuned, "num_thresholds": th_used,
+                   "tuning": {k: v.split("; thresholds ")[-1] for k, v in tuning.items()},
+                   **compare},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

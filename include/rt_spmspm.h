@@ -165,7 +165,8 @@ int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cu
  * RT_MODE_LEARNED) on the given device operands, `reps` timed runs each,
  * and records the fastest for the operands' matrix class in the tuning
  * cache (RTSPMSPM_TUNE_CACHE or ~/.cache/rtspmspm/tune.txt).  Later calls
- * with rt_opts.use_tuned = 1 use it.  desc receives a one-line summary.
+ * with rt_opts.use_tuned = 1 use it.  desc receives a one-line summary
+ * ending in "thresholds sym <5> num <5> learned_heavy <0|1>".
  * No counterpart in the reference (its pipeline is fixed at build time).   */
 int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, char* desc,
                 int64_t desc_len);

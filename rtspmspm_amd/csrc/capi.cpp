@@ -348,8 +348,14 @@ int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, ch
     }
     const std::string key = tune_key(a, b, mode);
     record_tuning(key, cand[best]);
+    std::string chosen = "; thresholds sym";
+    for (i64 v : cand[best].sym) chosen += " " + std::to_string(v);
+    chosen += " num";
+    for (i64 v : cand[best].num) chosen += " " + std::to_string(v);
+    chosen += " learned_heavy " + std::to_string(cand[best].learned_heavy);
     const std::string line = "class " + key + " best candidate " + std::to_string(best) + " (" +
-                             std::to_string(best_ms) + " ms); ms per candidate: " + report;
+                             std::to_string(best_ms) + " ms); ms per candidate: " + report +
+                             chosen;
     if (desc && desc_len > 0) {
       std::strncpy(desc, line.c_str(), desc_len - 1);
       desc[desc_len - 1] = '\0';

@@ -49,12 +49,12 @@ struct RunStats {
 };
 
 // Rows the symbolic pass handled with the LDS-bitmap kernel, with their
-// per-32768-column-tile nnz (used by the dense-tile numeric kernel).
+// per-tile (TILE_W columns) counts, used by the long-row numeric kernels.
 struct HeavyRows {
   DBuf<i32> list;  // all binned rows; the heavy ones start at `off`
   i32 off = 0, n = 0, nt = 1;
   DBuf<i32> tilecnt;  // n x nt: entries | nonzero bitmap words << 16 per tile
-  DBuf<u32> rowbits;  // n x nt x 1024 words: each long row's column set (may be empty)
+  DBuf<u32> rowbits;  // n x nt x TILE_W/32 words: each long row's column set (may be empty)
   DBuf<i32> next;     // row-slot counter of the symbolic long-row kernel
   DBuf<i32> slab;     // B.rows x (nt+1) tile boundaries of B's rows
 };
