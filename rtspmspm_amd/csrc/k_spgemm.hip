@@ -280,8 +280,12 @@ struct ChunkLds {
   OffT* off;   // [WG] B index of product p is off[e] + p
   float* av;   // [WG] A value of the entry
   i32* ex;     // [WG+1] first product of the entry; ex[nvalid] = tot
-  u16* mark;   // [WG/64][64*ROWS] per-wave zeroed owner marks
+  u16* mark;   // [WG/64][64*ROWS] per-wave zeroed owner marks (16-byte aligned)
   i32* scr;    // [WG/64] scan scratch
+  // With int32 offsets and values, off and av share one 8-byte word per
+  // entry (A value bits << 32 | offset), read with one LDS instruction:
+  // off then points at WG u64 words and av is unused.
+  static constexpr bool packs(bool val) { return val && sizeof(OffT) == 4; }
 };
 
 // f(col, prod) is called for every product: col = B column, prod = A*B value
@@ -304,8 +308,13 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
   const int ex = inc - len;
   s.ex[l] = ex;
   if (l == 0) s.ex[nvalid] = tot;
-  s.off[l] = start - (OffT)ex;
-  if constexpr (VAL) s.av[l] = av;
+  constexpr bool PACK = ChunkLds<OffT>::packs(VAL);
+  if constexpr (PACK) {
+    reinterpret_cast<u64*>(s.off)[l] = ((u64)__float_as_uint(av) << 32) | (u32)(start - (OffT)ex);
+  } else {
+    s.off[l] = start - (OffT)ex;
+    if constexpr (VAL) s.av[l] = av;
+  }
   __syncthreads();
   u16* wm = s.mark + w * SB;
   const int per = ((tot + NW - 1) / NW + WAVE - 1) & ~(WAVE - 1);
@@ -334,13 +343,29 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
       for (int r = 0; r < ROWS; ++r) {
         const int q = r * WAVE + lane;
         int m = wm[q];
-        wm[q] = 0;
         m = max(wave_max_scan(m), carry);
         carry = __builtin_amdgcn_readlane(m, WAVE - 1);
         const int e = m - 1;
         ok[r] = p0 + q < p1;
-        idx[r] = s.off[e] + (OffT)(p0 + q);
-        if constexpr (VAL) a[r] = s.av[e];
+        if constexpr (PACK) {
+          const u64 pk = reinterpret_cast<const u64*>(s.off)[e];
+          idx[r] = (OffT)(i32)(u32)pk + (OffT)(p0 + q);
+          a[r] = __uint_as_float((u32)(pk >> 32));
+        } else {
+          idx[r] = s.off[e] + (OffT)(p0 + q);
+          if constexpr (VAL) a[r] = s.av[e];
+        }
+      }
+      // Clear the sub-batch's marks, ROWS per lane in one store (the fence
+      // keeps the mark reads above before it).
+      wave_sync();
+      if constexpr (ROWS == 8) {
+        reinterpret_cast<uint4*>(wm)[lane] = make_uint4(0u, 0u, 0u, 0u);
+      } else if constexpr (ROWS == 4) {
+        reinterpret_cast<uint2*>(wm)[lane] = make_uint2(0u, 0u);
+      } else {
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) wm[r * WAVE + lane] = 0;
       }
       wave_sync();  // marks cleared before the next sub-batch writes them
       // Unconditional loads (lanes past the end read entry 0): a load under
@@ -978,7 +1003,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
   __shared__ i64 s_off[WG];
   __shared__ i32 s_ex[WG + 1];
   __shared__ float s_av[WG];
-  __shared__ u16 s_mark[4 * WG];
+  __shared__ __align__(16) u16 s_mark[4 * WG];
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   // The emit's word prefix reuses s_off (rewritten at every chunk start).
@@ -1291,8 +1316,10 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
   constexpr int WG = HWG;
   static_assert(TILE_W / 32 == WG, "one word of a dense tile per thread");
   __shared__ __align__(16) unsigned char s_raw[HSEG_LDS];
-  __shared__ OffT s_off[WG];
-  __shared__ float s_av[WG];
+  // chunk entries: WG offsets and WG values, or WG packed words (int32 offsets)
+  __shared__ __align__(16) unsigned char s_ent[WG * (sizeof(OffT) + 4)];
+  OffT* s_off = reinterpret_cast<OffT*>(s_ent);
+  float* s_av = reinterpret_cast<float*>(s_ent + WG * sizeof(OffT));
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
