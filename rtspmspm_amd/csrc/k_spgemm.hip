@@ -69,12 +69,24 @@ struct SplitEntries {
     c = col[i];
     v = val[i];
   }
+  // Entry base + j, with base uniform (a scalar base address, a lane offset).
+  template <class OffT>
+  __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float& v) const {
+    c = (col + base)[j];
+    v = (val + base)[j];
+  }
 };
 struct PairedEntries {
   const uint2* __restrict__ cv;
   template <class OffT>
   __device__ __forceinline__ void load(OffT i, i32& c, float& v) const {
     const uint2 x = cv[i];
+    c = (i32)x.x;
+    v = __uint_as_float(x.y);
+  }
+  template <class OffT>
+  __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float& v) const {
+    const uint2 x = (cv + base)[j];
     c = (i32)x.x;
     v = __uint_as_float(x.y);
   }
@@ -86,6 +98,10 @@ struct ColumnEntries {
   template <class OffT>
   __device__ __forceinline__ void load(OffT i, i32& c, float&) const {
     c = col[i];
+  }
+  template <class OffT>
+  __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float&) const {
+    c = (col + base)[j];
   }
 };
 
@@ -260,131 +276,249 @@ __device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo,
   return lo;
 }
 
-// Product enumeration by owner marks, for the long-row kernels.  A
-// chunk of up to WG entries A(i,k) -- entry l held by thread l as (run
-// length, first B index, A value) -- has products 0..tot-1, entry e owning
-// [ex[e], ex[e] + len[e]).  After one block scan the waves work
-// independently: wave w takes an equal contiguous slice of the products and
-// walks it in sub-batches of 64*ROWS (lane l: products 64r + l of the
-// sub-batch, so each load instruction reads 64 consecutive products).  For a
-// sub-batch the wave marks, in its own zeroed u16 array, the start of every
-// nonempty entry that begins inside it (e+1 at ex[e]); each product's entry
-// is then the running maximum of the marks (a DPP max-scan over the wave),
-// seeded with the entry that covers the sub-batch's first product.  Per
-// product: one mark read, two LDS reads of the entry's offset and value, two
-// B loads; ROWS products per lane are in flight, and no wave waits for
-// another inside the chunk.  OffT is the B index type (int32 when nnz(B) <
-// 2^31).  Four barriers per chunk (two in the scan).
+// Product enumeration for the long-row kernels.  A chunk of up to WG
+// entries A(i,k) -- entry l held by thread l as (run length, first B index,
+// A value) -- has products 0..tot-1.  One block scan gives every entry both
+// its first product and its first 64-product block, and the chunk is walked
+// in one of two ways, chosen per chunk:
+//
+// * Blocks, when whole 64-product blocks leave at most a quarter of the
+//   lanes idle (long runs: the usual case in long rows).  Each run is cut
+//   into blocks of 64 consecutive products of one entry, and wave w takes
+//   an equal contiguous range of blocks.  For a batch of 64 blocks, lane j
+//   finds the owner of block j by binary search over the entries' first
+//   blocks and computes the block's first B index, length and A value; the
+//   wave then walks the batch ROWS blocks at a time, lane l taking product
+//   l of each block, with the block's fields read from lane j (v_readlane):
+//   no per-product entry lookup, and the B loads use a scalar base.
+// * Owner marks, otherwise (short runs).  Wave w takes an equal contiguous
+//   slice of the products and walks it in sub-batches of 64*ROWS (lane l:
+//   products 64r + l).  For a sub-batch the wave marks, in its own zeroed
+//   u16 array, the start of every nonempty entry that begins inside it (e+1
+//   at ex[e]); each product's entry is the running maximum of the marks (a
+//   DPP max-scan over the wave), seeded with the entry that covers the
+//   sub-batch's first product.
+//
+// Either way a lane issues its ROWS B loads before using any (loads under a
+// branch would make the compiler wait for each in turn) and hands the ROWS
+// products to f at once, f(c, p, ok) with arrays of ROWS: c the B columns,
+// p the products A*B (0 when VAL is false), ok whether the lane holds a
+// product in that row (c is unspecified where it does not).  No wave waits
+// for another inside the chunk; four barriers per chunk (two in the scan).
+// OffT is the B index type (int32 when nnz(B) < 2^31).  A chunk's products
+// must number fewer than 2^30.
 template <class OffT>
 struct ChunkLds {
-  OffT* off;   // [WG] B index of product p is off[e] + p
+  OffT* off;   // [WG] B index of the entry's product p is off[e] + p (marks) or off[e] + 64 b + lane (blocks)
   float* av;   // [WG] A value of the entry
-  i32* ex;     // [WG+1] first product of the entry; ex[nvalid] = tot
-  u16* mark;   // [WG/64][64*ROWS] per-wave zeroed owner marks (16-byte aligned)
-  i32* scr;    // [WG/64] scan scratch
+  i32* ex;     // [WG+1] first product (marks) or first block (blocks) of the entry
+  u16* mark;   // [WG/64][64*ROWS] per-wave zeroed owner marks (16-byte aligned);
+               // block mode keeps each entry's product end (64 x first block
+               // + run length) in its first WG*4 bytes and zeroes them after
+  u64* scr;    // [WG/64] scan scratch
   // With int32 offsets and values, off and av share one 8-byte word per
   // entry (A value bits << 32 | offset), read with one LDS instruction:
   // off then points at WG u64 words and av is unused.
   static constexpr bool packs(bool val) { return val && sizeof(OffT) == 4; }
 };
 
-// f(col, prod) is called for every product: col = B column, prod = A*B value
-// (0 when VAL is false).  Returns the chunk's product count (uniform).
+// v read from lane j (j uniform); the result is uniform.
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int j) {
+  if constexpr (sizeof(T) == 8) {
+    const u64 u = (u64)v;
+    const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)u, j);
+    const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(u >> 32), j);
+    return (T)(((u64)hi << 32) | lo);
+  } else {
+    return (T)__builtin_amdgcn_readlane((int)v, j);
+  }
+}
+__device__ __forceinline__ float lane_value(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Returns the chunk's product count (uniform).
 template <int WG, int ROWS, bool VAL, class OffT, class Src, class F>
 __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int nvalid,
                                               const Src& src, const ChunkLds<OffT>& s, F&& f) {
   static_assert(WG % WAVE == 0 && WG <= 65535, "marks are u16 entry indices + 1");
+  static_assert(WAVE % ROWS == 0 && ROWS >= 2, "blocks are walked ROWS at a time; the entry "
+                                               "ends fit the mark array");
   constexpr int NW = WG / WAVE;
-  constexpr int SB = ROWS * WAVE;  // products per sub-batch
+  constexpr bool PACK = ChunkLds<OffT>::packs(VAL);
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1);
   const int w = __builtin_amdgcn_readfirstlane(l / WAVE);
-  int tot;
-  const int inc = block_scan_incl<WG>(len, s.scr, &tot);
+  const int nb = (len + WAVE - 1) / WAVE;
+  u64 both;
+  const u64 inc = block_scan_incl<WG>(((u64)(u32)len << 32) | (u32)nb, s.scr, &both);
+  const int tot = (int)(both >> 32), tb = (int)(u32)both;
   if (tot == 0) {
     __syncthreads();  // every wave has read s.scr before the next scan writes it
     return 0;
   }
-  const int ex = inc - len;
-  s.ex[l] = ex;
-  if (l == 0) s.ex[nvalid] = tot;
-  constexpr bool PACK = ChunkLds<OffT>::packs(VAL);
-  if constexpr (PACK) {
-    reinterpret_cast<u64*>(s.off)[l] = ((u64)__float_as_uint(av) << 32) | (u32)(start - (OffT)ex);
-  } else {
-    s.off[l] = start - (OffT)ex;
-    if constexpr (VAL) s.av[l] = av;
-  }
-  __syncthreads();
-  u16* wm = s.mark + w * SB;
-  const int per = ((tot + NW - 1) / NW + WAVE - 1) & ~(WAVE - 1);
-  const int pb = w * per, pe = min(tot, pb + per);
-  if (pb < pe) {  // wave-uniform
-    int lo = 0, hi = nvalid - 1;  // last entry with ex <= pb: the owner of pb
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s.ex[mid] <= pb) lo = mid; else hi = mid - 1;
+  const bool blocks = (i64)tot * 4 >= (i64)tb * (3 * WAVE);
+  if (blocks) {
+    const int bex = (int)(u32)inc - nb;
+    i32* s_end = reinterpret_cast<i32*>(s.mark);
+    s.ex[l] = bex;
+    s_end[l] = bex * WAVE + len;
+    const OffT offp = start - (OffT)bex * WAVE;
+    if constexpr (PACK) {
+      reinterpret_cast<u64*>(s.off)[l] = ((u64)__float_as_uint(av) << 32) | (u32)offp;
+    } else {
+      s.off[l] = offp;
+      if constexpr (VAL) s.av[l] = av;
     }
-    int carry = lo + 1;  // owner of the sub-batch's first product, plus one
-    for (int p0 = pb; p0 < pe; p0 += SB) {
-      const int p1 = min(pe, p0 + SB);
-      // Mark the nonempty entries that start in (p0, p1).
-      for (int c0 = carry; ; c0 += WAVE) {
-        const int c = c0 + lane;
-        const int xc = c < nvalid ? s.ex[c] : INT_MAX;
-        if (xc < p1 && s.ex[c + 1] > xc) wm[xc - p0] = (u16)(c + 1);
-        if (!__any(xc < p1)) break;
+    __syncthreads();
+    const int per = (tb + NW - 1) / NW;
+    const int pb = w * per, pe = min(tb, pb + per);
+    int lo = 0;  // owner of the batch's first block, or an entry before it
+    for (int b0 = pb; b0 < pe; b0 += WAVE) {
+      const int nblk = min(WAVE, pe - b0);
+      const int b = b0 + min(lane, nblk - 1);
+      int e = lo, hi = nvalid - 1;  // last entry whose first block is <= b
+      while (e < hi) {
+        const int mid = (e + hi + 1) >> 1;
+        if (s.ex[mid] <= b) e = mid; else hi = mid - 1;
       }
-      wave_sync();
-      OffT idx[ROWS];
-      float a[ROWS];
-      bool ok[ROWS];
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        const int q = r * WAVE + lane;
-        int m = wm[q];
-        m = max(wave_max_scan(m), carry);
-        carry = __builtin_amdgcn_readlane(m, WAVE - 1);
-        const int e = m - 1;
-        ok[r] = p0 + q < p1;
-        if constexpr (PACK) {
-          const u64 pk = reinterpret_cast<const u64*>(s.off)[e];
-          idx[r] = (OffT)(i32)(u32)pk + (OffT)(p0 + q);
-          a[r] = __uint_as_float((u32)(pk >> 32));
-        } else {
-          idx[r] = s.off[e] + (OffT)(p0 + q);
-          if constexpr (VAL) a[r] = s.av[e];
-        }
-      }
-      // Clear the sub-batch's marks, ROWS per lane in one store (the fence
-      // keeps the mark reads above before it).
-      wave_sync();
-      if constexpr (ROWS == 8) {
-        reinterpret_cast<uint4*>(wm)[lane] = make_uint4(0u, 0u, 0u, 0u);
-      } else if constexpr (ROWS == 4) {
-        reinterpret_cast<uint2*>(wm)[lane] = make_uint2(0u, 0u);
+      lo = __builtin_amdgcn_readlane(e, WAVE - 1);
+      OffT bs;
+      float ae = 0.f;
+      if constexpr (PACK) {
+        const u64 pk = reinterpret_cast<const u64*>(s.off)[e];
+        bs = (OffT)(i32)(u32)pk;
+        ae = __uint_as_float((u32)(pk >> 32));
       } else {
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) wm[r * WAVE + lane] = 0;
+        bs = s.off[e];
+        if constexpr (VAL) ae = s.av[e];
       }
-      wave_sync();  // marks cleared before the next sub-batch writes them
-      // Unconditional loads (lanes past the end read entry 0): a load under
-      // a branch makes the compiler wait for every earlier load before the
-      // next one, which would serialise the rows.
-      i32 c[ROWS];
-      float v[ROWS];
+      bs += (OffT)b * WAVE;                                    // the block's first B index
+      const int rem = lane < nblk ? s_end[e] - b * WAVE : 0;  // its products (> 64: all)
+      for (int j0 = 0; j0 < nblk; j0 += ROWS) {
+        i32 c[ROWS];
+        float v[ROWS], p[ROWS];
+        bool ok[ROWS];
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        const OffT li = ok[r] ? idx[r] : (OffT)0;
-        src.load(li, c[r], v[r]);
+        for (int r = 0; r < ROWS; ++r) {
+          ok[r] = lane < __builtin_amdgcn_readlane(rem, j0 + r);
+          src.load_at(lane_value(bs, j0 + r), ok[r] ? lane : 0, c[r], v[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) p[r] = VAL ? lane_value(ae, j0 + r) * v[r] : 0.f;
+        f(c, p, ok);
       }
+    }
+  } else {
+    constexpr int SB = ROWS * WAVE;  // products per sub-batch
+    const int ex = (int)(inc >> 32) - len;
+    s.ex[l] = ex;
+    if (l == 0) s.ex[nvalid] = tot;
+    if constexpr (PACK) {
+      reinterpret_cast<u64*>(s.off)[l] = ((u64)__float_as_uint(av) << 32) | (u32)(start - (OffT)ex);
+    } else {
+      s.off[l] = start - (OffT)ex;
+      if constexpr (VAL) s.av[l] = av;
+    }
+    __syncthreads();
+    u16* wm = s.mark + w * SB;
+    const int per = ((tot + NW - 1) / NW + WAVE - 1) & ~(WAVE - 1);
+    const int pb = w * per, pe = min(tot, pb + per);
+    if (pb < pe) {  // wave-uniform
+      int lo = 0, hi = nvalid - 1;  // last entry with ex <= pb: the owner of pb
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s.ex[mid] <= pb) lo = mid; else hi = mid - 1;
+      }
+      int carry = lo + 1;  // owner of the sub-batch's first product, plus one
+      for (int p0 = pb; p0 < pe; p0 += SB) {
+        const int p1 = min(pe, p0 + SB);
+        // Mark the nonempty entries that start in (p0, p1).
+        for (int c0 = carry; ; c0 += WAVE) {
+          const int c = c0 + lane;
+          const int xc = c < nvalid ? s.ex[c] : INT_MAX;
+          if (xc < p1 && s.ex[c + 1] > xc) wm[xc - p0] = (u16)(c + 1);
+          if (!__any(xc < p1)) break;
+        }
+        wave_sync();
+        OffT idx[ROWS];
+        float a[ROWS];
+        bool ok[ROWS];
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r)
-        if (ok[r]) f(c[r], VAL ? a[r] * v[r] : 0.f);
+        for (int r = 0; r < ROWS; ++r) {
+          const int q = r * WAVE + lane;
+          int m = wm[q];
+          m = max(wave_max_scan(m), carry);
+          carry = __builtin_amdgcn_readlane(m, WAVE - 1);
+          const int e = m - 1;
+          ok[r] = p0 + q < p1;
+          if constexpr (PACK) {
+            const u64 pk = reinterpret_cast<const u64*>(s.off)[e];
+            idx[r] = (OffT)(i32)(u32)pk + (OffT)(p0 + q);
+            a[r] = __uint_as_float((u32)(pk >> 32));
+          } else {
+            idx[r] = s.off[e] + (OffT)(p0 + q);
+            a[r] = 0.f;
+            if constexpr (VAL) a[r] = s.av[e];
+          }
+        }
+        // Clear the sub-batch's marks, ROWS per lane in one store (the fence
+        // keeps the mark reads above before it).
+        wave_sync();
+        if constexpr (ROWS == 8) {
+          reinterpret_cast<uint4*>(wm)[lane] = make_uint4(0u, 0u, 0u, 0u);
+        } else if constexpr (ROWS == 4) {
+          reinterpret_cast<uint2*>(wm)[lane] = make_uint2(0u, 0u);
+        } else {
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) wm[r * WAVE + lane] = 0;
+        }
+        wave_sync();  // marks cleared before the next sub-batch writes them
+        i32 c[ROWS];
+        float v[ROWS], p[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          const OffT li = ok[r] ? idx[r] : (OffT)0;
+          src.load(li, c[r], v[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) p[r] = VAL ? a[r] * v[r] : 0.f;
+        f(c, p, ok);
+      }
     }
   }
   __syncthreads();  // the chunk's entries are read before the next chunk writes them
+  if (blocks) reinterpret_cast<i32*>(s.mark)[l] = 0;  // marks start the next chunk zeroed
   return tot;
+}
+
+// Batched LDS FP32 accumulation of ROWS products per lane: the first
+// compare-and-swap of every row is issued before any result is checked, so
+// their LDS round trips overlap; a row whose swap lost a race retries alone.
+// old[r] is the word's value read before (any earlier value is correct, only
+// slower).
+template <int ROWS>
+__device__ __forceinline__ void lds_add_f32_rows(float* base, const int (&x)[ROWS],
+                                                 const float (&p)[ROWS], const bool (&ok)[ROWS],
+                                                 const u32 (&old)[ROWS]) {
+  u32* q = reinterpret_cast<u32*>(base);
+  u32 got[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    got[r] = old[r];
+    if (ok[r]) got[r] = atomicCAS(&q[x[r]], old[r], __float_as_uint(__uint_as_float(old[r]) + p[r]));
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    if (ok[r] && got[r] != old[r]) {
+      u32 cur = got[r], assumed;
+      do {
+        assumed = cur;
+        cur = atomicCAS(&q[x[r]], assumed, __float_as_uint(__uint_as_float(assumed) + p[r]));
+      } while (cur != assumed);
+    }
+  }
 }
 
 // LDS-hash row kernel.  A group of G lanes owns one output row and a table of
@@ -801,7 +935,7 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
   __shared__ OffT s_off[WG];
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
-  __shared__ i32 s_scr[WG / WAVE];
+  __shared__ u64 s_scr[WG / WAVE];
   __shared__ i64 s_scr64[WG / WAVE];
   const ChunkLds<OffT> cl{s_off, nullptr, s_ex, s_mark, s_scr};
   const ColumnEntries src{bcol};
@@ -833,9 +967,15 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
           len = sk[tw1] - s0;
         }
         const int nvalid = (int)min((i64)WG, a1 - e0);
-        chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl, [&](i32 c, float) {
-          const int x = (int)((i64)c - win);  // in [0, wcols) by the slab bounds
-          lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
+        chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl,
+                                        [&](const auto& c, const auto&, const auto& ok) {
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) {
+            if (ok[r]) {
+              const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
+              lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
+            }
+          }
         });
       }
       const int ntw = tw1 - tw0;
@@ -911,10 +1051,17 @@ __device__ __forceinline__ int tile_chunk(int len, i64 start, float av, i64 cbas
                                           const float* __restrict__ bval, float* s_vals,
                                           u32* s_bits, const ChunkLds<i64>& cl, int nvalid) {
   return chunk_products<WG, 4, true>(len, start, av, nvalid, SplitEntries{bcol, bval}, cl,
-                                     [&](i32 c, float p) {
-    const int x = (int)(c - cbase);
-    lds_add_f32(&s_vals[x], p);
-    lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
+                                     [&](const auto& c, const auto& p, const auto& ok) {
+    int x[4];
+    u32 old[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = ok[r] ? (int)(c[r] - cbase) : 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) old[r] = __float_as_uint(s_vals[x[r]]);
+    lds_add_f32_rows<4>(s_vals, x, p, ok, old);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (ok[r]) lds_or_u32(&s_bits[x[r] >> 5], 1u << (x[r] & 31));
   });
 }
 
@@ -1005,10 +1152,11 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
   __shared__ float s_av[WG];
   __shared__ __align__(16) u16 s_mark[4 * WG];
   __shared__ i32 s_scr[WG / WAVE];
+  __shared__ u64 s_cscr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   // The emit's word prefix reuses s_off (rewritten at every chunk start).
   i32* s_pfx = reinterpret_cast<i32*>(s_off);
-  const ChunkLds<i64> cl{s_off, s_av, s_ex, s_mark, s_scr};
+  const ChunkLds<i64> cl{s_off, s_av, s_ex, s_mark, s_cscr};
   const int l = threadIdx.x;
   for (int d = l; d < TILE_W; d += WG) s_vals[d] = 0.f;
   for (int d = l; d < 4 * WG; d += WG) s_mark[d] = 0;
@@ -1323,9 +1471,10 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ i32 s_scr[WG / WAVE];
+  __shared__ u64 s_cscr[WG / WAVE];
   __shared__ i32 s_tc[TC_LDS];
   float* s_acc = reinterpret_cast<float*>(s_raw);
-  const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_scr};
+  const ChunkLds<OffT> cl{s_off, s_av, s_ex, s_mark, s_cscr};
   const PairedEntries src{bcv};
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
@@ -1413,14 +1562,51 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         dz0 = n;  // the output re-zeroes [0, n)
         dz1 = max(dz1, end);
       }
-      auto add_dense = [&](i32 c, float p) { lds_add_f32(&s_acc[(int)(c - cbase)], p); };
-      auto add_rank = [&](i32 c, float p) {
-        const int x = (int)(c - cbase);
-        const u64 gg = s_gt[x >> 10];  // first slot << 32 | nonzero-word mask
-        const int slot = (int)(gg >> 32) + __popc((u32)gg & ((1u << ((x >> 5) & 31)) - 1u));
-        const u64 e = s_cwd[slot];  // entry prefix << 47 | word index << 32 | bits
-        const int rank = (int)(e >> 47) + __popc((u32)e & ((1u << (x & 31)) - 1u));
-        lds_add_f32(&s_acc[rank], p);
+      // Batched accumulation (every LDS stage of the ROWS products is
+      // issued before the next stage waits): dense steps index the
+      // accumulators by column; compressed steps by rank, two LDS reads
+      // away (group -> staged word).
+      auto add_dense = [&](const auto& c, const auto& p, const auto& ok) {
+        int x[ROWS];
+        u32 old[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) x[r] = ok[r] ? (int)(c[r] - cbase) : 0;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) old[r] = __float_as_uint(s_acc[x[r]]);
+        lds_add_f32_rows<ROWS>(s_acc, x, p, ok, old);
+      };
+      auto add_rank = [&](const auto& c, const auto& p, const auto& ok) {
+        // In halves of H rows (register pressure).
+        constexpr int H = ROWS / 2;
+#pragma unroll
+        for (int h0 = 0; h0 < ROWS; h0 += H) {
+          int x[H], rank[H];
+          u64 gg[H], e[H];
+          u32 old[H];
+          float ph[H];
+          bool okh[H];
+#pragma unroll
+          for (int r = 0; r < H; ++r) {
+            okh[r] = ok[h0 + r];
+            ph[r] = p[h0 + r];
+            x[r] = okh[r] ? (int)(c[h0 + r] - cbase) : 0;
+          }
+#pragma unroll
+          for (int r = 0; r < H; ++r) gg[r] = s_gt[x[r] >> 10];  // first slot << 32 | nonzero-word mask
+#pragma unroll
+          for (int r = 0; r < H; ++r) {
+            const int slot = (int)(gg[r] >> 32) +
+                             __popc((u32)gg[r] & ((1u << ((x[r] >> 5) & 31)) - 1u));
+            e[r] = s_cwd[slot];  // entry prefix << 47 | word index << 32 | bits
+          }
+#pragma unroll
+          for (int r = 0; r < H; ++r)
+            rank[r] = okh[r] ? (int)(e[r] >> 47) + __popc((u32)e[r] & ((1u << (x[r] & 31)) - 1u))
+                             : 0;
+#pragma unroll
+          for (int r = 0; r < H; ++r) old[r] = __float_as_uint(s_acc[rank[r]]);
+          lds_add_f32_rows<H>(s_acc, rank, ph, okh, old);
+        }
       };
       if (inreg) {
         for (int c = 0; c < nch; ++c) {
@@ -1770,11 +1956,11 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
       // int32 B offsets (and eight products per lane in flight) when B's
       // entries are indexable by int32.
       if (heavy.rowbits.p && B.nnz < ((i64)1 << 31))
-        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 2, 8>), dim3(grid), dim3(HWG), 0, s, rs,
+        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 1, 8>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
                            A.val, B.rp, pairs.p, bbase, slab, C.rp, C.col, C.val, next.p);
       else if (heavy.rowbits.p)
-        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 4, 4>), dim3(grid), dim3(HWG), 0, s, rs,
+        hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 2, 4>), dim3(grid), dim3(HWG), 0, s, rs,
                            heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
                            A.val, B.rp, pairs.p, bbase, slab, C.rp, C.col, C.val, next.p);
       else
