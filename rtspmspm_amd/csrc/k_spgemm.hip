@@ -303,7 +303,11 @@ __device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo,
 // branch would make the compiler wait for each in turn) and hands the ROWS
 // products to f at once, f(c, p, ok) with arrays of ROWS: c the B columns,
 // p the products A*B (0 when VAL is false), ok whether the lane holds a
-// product in that row (c is unspecified where it does not).  No wave waits
+// product in that row.  Where it does not, c is the column of another
+// product of the chunk (so within the columns the chunk's runs cover), and
+// f should still use it unconditionally: a load whose value is used only
+// under `ok` is sunk into the branch by the compiler and then waited for
+// alone, which costs the round a second memory latency.  No wave waits
 // for another inside the chunk; four barriers per chunk (two in the scan).
 // OffT is the B index type (int32 when nnz(B) < 2^31).  A chunk's products
 // must number fewer than 2^30.
@@ -453,12 +457,15 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
           carry = __builtin_amdgcn_readlane(m, WAVE - 1);
           const int e = m - 1;
           ok[r] = p0 + q < p1;
+          // Lanes past the sub-batch hold its last product's owner: point
+          // them at that product.
+          const int pq = min(p0 + q, p1 - 1);
           if constexpr (PACK) {
             const u64 pk = reinterpret_cast<const u64*>(s.off)[e];
-            idx[r] = (OffT)(i32)(u32)pk + (OffT)(p0 + q);
+            idx[r] = (OffT)(i32)(u32)pk + (OffT)pq;
             a[r] = __uint_as_float((u32)(pk >> 32));
           } else {
-            idx[r] = s.off[e] + (OffT)(p0 + q);
+            idx[r] = s.off[e] + (OffT)pq;
             a[r] = 0.f;
             if constexpr (VAL) a[r] = s.av[e];
           }
@@ -478,10 +485,7 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
         i32 c[ROWS];
         float v[ROWS], p[ROWS];
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-          const OffT li = ok[r] ? idx[r] : (OffT)0;
-          src.load(li, c[r], v[r]);
-        }
+        for (int r = 0; r < ROWS; ++r) src.load(idx[r], c[r], v[r]);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) p[r] = VAL ? a[r] * v[r] : 0.f;
         f(c, p, ok);
@@ -916,7 +920,10 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
 // own products, so each product is gathered once.  Records for every tile of
 // the row its entry count and its count of nonzero bitmap words, packed as
 // entries | words << 16 (the long-row numeric kernels' step plan and output
-// offsets), and, when rowbits is set, the row's bitmap.
+// offsets), and, when rowbits is set, the row's bitmap in compressed form:
+// rowmask gets per tile 16 masks of nonzero words (one per 32-word group),
+// rowbits the nonzero words themselves, packed in column order.  About a
+// quarter of an R-MAT long row's bitmap words are nonzero.
 constexpr int SYM_WIN_TILES = 32;
 constexpr int SYM_WIN = SYM_WIN_TILES * TILE_W;
 
@@ -926,9 +933,11 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
     const i32* __restrict__ acol, const i64* __restrict__ brp,
     const i32* __restrict__ bcol, const i64* __restrict__ bbase, const i32* __restrict__ slab,
     i64 ncols, i32 nt, i32* __restrict__ tilecnt, i64* __restrict__ out_nnz,
-    u32* __restrict__ rowbits, i32* __restrict__ next) {
+    u32* __restrict__ rowbits, u32* __restrict__ rowmask, i32* __restrict__ next) {
   constexpr int WG = HWG;
   constexpr int WW = SYM_WIN / 32;  // bitmap words per window
+  constexpr int GPT = TILE_W / 32 / 32;  // 32-word groups per tile
+  static_assert(WW / 32 == WG, "one 32-word group of the window per thread");
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   __shared__ u32 s_bits[WW];
@@ -937,10 +946,10 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ u64 s_scr[WG / WAVE];
   __shared__ i64 s_scr64[WG / WAVE];
+  __shared__ i32 s_scr32[WG / WAVE];
   const ChunkLds<OffT> cl{s_off, nullptr, s_ex, s_mark, s_scr};
   const ColumnEntries src{bcol};
   const int l = threadIdx.x;
-  const int lane = l & (WAVE - 1), w = l / WAVE;
   for (int d = l; d < ROWS * WG; d += WG) s_mark[d] = 0;
   __shared__ i32 s_slot;
   for (i64 it = next_slot(next, &s_slot); it < nlist; it = next_slot(next, &s_slot)) {
@@ -948,6 +957,7 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
     const i64 a0 = arp[row], a1 = arp[row + 1];
     const i64 bb = BATCH ? bbase[row] : 0;
     i64 total = 0;
+    int wcarry = 0;  // nonzero words of the row stored so far
     for (int tw0 = 0; tw0 < nt; tw0 += SYM_WIN_TILES) {
       const int tw1 = min(nt, tw0 + SYM_WIN_TILES);
       const i64 win = (i64)tw0 * TILE_W;
@@ -971,31 +981,66 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
                                         [&](const auto& c, const auto&, const auto& ok) {
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) {
-            if (ok[r]) {
-              const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
-              lds_or_u32(&s_bits[x >> 5], 1u << (x & 31));
-            }
+            const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
+            const u32 bit = ok[r] ? 1u << (x & 31) : 0u;
+            if (bit) lds_or_u32(&s_bits[x >> 5], bit);
           }
         });
       }
       const int ntw = tw1 - tw0;
-      if (rowbits) {  // keep the row's column set (padded to whole tiles) for the numeric pass
-        u32* dst = rowbits + it * ((i64)nt * (TILE_W / 32)) + win / 32;
-        for (int d = l; d < ntw * (TILE_W / 32); d += WG) dst[d] = d < nd ? s_bits[d] : 0u;
-      }
-      i64 wsum = 0;
-      for (int t = w; t < ntw; t += WG / WAVE) {
-        int c = 0;  // entries | nonzero words << 16 (entries <= 16384)
-        const int dend = min((t + 1) * (TILE_W / 32), nd);
-        for (int d = t * (TILE_W / 32) + lane; d < dend; d += WAVE) {
-          const u32 x = s_bits[d];
-          c += __popc(x) + (x != 0u ? 0x10000 : 0);
+      // Thread l owns the window's 32-word group l (words past nd are stale
+      // and count as zero): its nonzero-word mask and entry count.  The 16
+      // groups of a tile sit on 16 consecutive lanes.
+      u32 gw[32];
+      u32 gmask = 0;
+      int gent = 0;
+      {
+        const int lim = nd - l * 32;
+        const uint4* q = reinterpret_cast<const uint4*>(s_bits) + l * 8;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint4 x = q[j];
+          gw[4 * j] = x.x;
+          gw[4 * j + 1] = x.y;
+          gw[4 * j + 2] = x.z;
+          gw[4 * j + 3] = x.w;
         }
-        c = seg_reduce_sum<WAVE>(c);
-        if (lane == 0) tilecnt[it * nt + tw0 + t] = c;
-        wsum += c & 0xffff;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+          if (j >= lim) gw[j] = 0u;
+          gmask |= (gw[j] != 0u ? 1u : 0u) << j;
+          gent += __popc(gw[j]);
+        }
       }
-      total += block_reduce_sum<WG>(lane == 0 ? wsum : (i64)0, s_scr64);
+      const int gcnt = __popc(gmask);
+      // Tile counts: entries | nonzero words << 16 (entries <= 16384).
+      const int tent = seg_reduce_sum<GPT>(gent), twd = seg_reduce_sum<GPT>(gcnt);
+      const int tl = l / GPT;
+      if ((l & (GPT - 1)) == 0 && tl < ntw) tilecnt[it * nt + tw0 + tl] = tent | (twd << 16);
+      if (rowbits) {
+        // Keep the row's column set for the numeric pass, compressed: per
+        // tile the 16 group masks, and the nonzero words packed in column
+        // order (a tile's words start after those of the row's earlier
+        // tiles).  The words are packed in place in s_bits (every thread
+        // holds its own; the scan's barriers order the reads before the
+        // writes) and written with coalesced stores.  Both are read once by
+        // the numeric pass: non-temporal stores keep them from evicting B's
+        // rows from L2.
+        int wtot;
+        const int gpos = block_scan_incl<WG>(gcnt, s_scr32, &wtot) - gcnt;
+        if (tl < ntw)
+          __builtin_nontemporal_store(gmask, &rowmask[it * ((i64)nt * GPT) + (i64)tw0 * GPT + l]);
+        int k = gpos;
+#pragma unroll
+        for (int j = 0; j < 32; ++j)
+          if (gw[j] != 0u) s_bits[k++] = gw[j];
+        __syncthreads();
+        u32* dst = rowbits + it * ((i64)nt * (TILE_W / 32)) + wcarry;
+        for (int q = l; q < wtot; q += WG) __builtin_nontemporal_store(s_bits[q], &dst[q]);
+        wcarry += wtot;
+      }
+      // (ends with a barrier: s_bits is read before the next window clears it)
+      total += block_reduce_sum<WG>((i64)gent, s_scr64);
     }
     if (l == 0) out_nnz[row] = total;
     __syncthreads();
@@ -1055,13 +1100,15 @@ __device__ __forceinline__ int tile_chunk(int len, i64 start, float av, i64 cbas
     int x[4];
     u32 old[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) x[r] = ok[r] ? (int)(c[r] - cbase) : 0;
+    for (int r = 0; r < 4; ++r) x[r] = (int)(c[r] - cbase);  // in the tile for every lane
 #pragma unroll
     for (int r = 0; r < 4; ++r) old[r] = __float_as_uint(s_vals[x[r]]);
     lds_add_f32_rows<4>(s_vals, x, p, ok, old);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (ok[r]) lds_or_u32(&s_bits[x[r] >> 5], 1u << (x[r] & 31));
+    for (int r = 0; r < 4; ++r) {
+      const u32 bit = ok[r] ? 1u << (x[r] & 31) : 0u;
+      if (bit) lds_or_u32(&s_bits[x[r] >> 5], bit);
+    }
   });
 }
 
@@ -1271,8 +1318,9 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
 // ---------------------------------------------------------------------------
 // Segmented long-row numeric kernel (exact method, row bitmaps available).
 //
-// The symbolic pass stored each long row's column set as a bitmap, and per
-// tile its entry count and nonzero-word count.  With them the row is
+// The symbolic pass stored each long row's column set as a compressed bitmap
+// (group masks and packed nonzero words), and per tile its entry count and
+// nonzero-word count.  With them the row is
 // processed in few steps.  A "compressed" step covers up to SEG_MAXT
 // consecutive tiles (2^20 columns) whose accumulators, staged words and
 // group table fit the 64 KB accumulator region: only the window's nonzero
@@ -1364,57 +1412,58 @@ __device__ __forceinline__ SegStep seg_plan(const i32* s_tc, const i32* tc, int 
   return p;
 }
 
-// Stages the nonzero words of compressed step [t, t1) (layout above).
-// Thread l takes the window's 32-word group g = l (then l + WG): the 16
-// groups of a tile sit on 16 consecutive lanes, whose scan of (nonzero
-// words, entries), added to the tile's own prefixes (lane j of every wave
-// computes tile j's), gives each group's first slot and entry prefix.  Ends
-// with a barrier.
+// Stages compressed step [t, t1) (layout above) from the row's compressed
+// bitmap: pw = the step's nw packed nonzero words, mk = its tiles' group
+// masks (16 per tile).  The words are copied, coalesced, to s_r (the
+// accumulator region's first nw words: nw <= n), with every load of the
+// copy and of the masks issued before the first is used; then thread l
+// takes the window's 32-word group g = l (then l + WG): the 16 groups of a
+// tile sit on 16 consecutive lanes, whose scans of (nonzero words,
+// entries), added to the tile's own prefixes (lane j of every wave computes
+// tile j's from the tile counts), give each group's first slot and entry
+// prefix.  Two barriers; s_r is left dirty.  The copied words and masks are
+// read once: non-temporal loads keep them from evicting B's rows from L2.
 template <int WG>
-__device__ __forceinline__ void stage_words(const u32* __restrict__ src, const i32* s_tc,
-                                            const i32* tc, int nt, int t, int t1, u64* s_cwd,
-                                            u64* s_gt) {
+__device__ __forceinline__ void stage_packed(const u32* __restrict__ pw,
+                                             const u32* __restrict__ mk, const i32* s_tc,
+                                             const i32* tc, int nt, int t, int t1, int nw,
+                                             u32* s_r, u64* s_cwd, u64* s_gt) {
+  static_assert(SEG_MAXT * SEG_GPT <= 2 * WG, "two groups per thread");
+  constexpr int CU = 4;  // copy loads in flight per thread
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1);
+  const int ngroups = (t1 - t) * SEG_GPT;
+  const u32 m0 = __builtin_nontemporal_load(&mk[min(l, ngroups - 1)]);
+  for (int q0 = 0; q0 < nw; q0 += CU * WG) {  // uniform; unconditional (clamped) loads
+    u32 r[CU];
+#pragma unroll
+    for (int k = 0; k < CU; ++k) r[k] = __builtin_nontemporal_load(&pw[min(q0 + k * WG + l, nw - 1)]);
+#pragma unroll
+    for (int k = 0; k < CU; ++k)
+      if (q0 + k * WG + l < nw) s_r[q0 + k * WG + l] = r[k];
+  }
   const int v = lane < t1 - t ? tile_count(s_tc, tc, nt, t + lane) : 0;
   const int c = tile_entries(v), wd = tile_words(v);
-  const int pc = seg_scan_incl<WAVE>(c) - c, pw = seg_scan_incl<WAVE>(wd) - wd;
-  const int ngroups = (t1 - t) * SEG_GPT;
-  for (int g0 = 0; g0 < ngroups; g0 += WG) {
+  const int pc = seg_scan_incl<WAVE>(c) - c, pw0 = seg_scan_incl<WAVE>(wd) - wd;
+  __syncthreads();
+  for (int g0 = 0; g0 < ngroups; g0 += WG) {  // uniform
     const int g = g0 + l;
-    u32 wv[32];
-    u32 mask = 0;
+    const u32 mask = g < ngroups ? (g0 ? __builtin_nontemporal_load(&mk[g]) : m0) : 0u;
+    const int cnt = __popc(mask);
+    const int tj = (g / SEG_GPT) & (WAVE - 1);  // the group's tile in the step (< 64)
+    const int slot0 = __shfl(pw0, tj) + seg_scan_incl<SEG_GPT>(cnt) - cnt;
     int ent = 0;
+    for (int q = 0; q < cnt; ++q) ent += __popc(s_r[slot0 + q]);
+    u32 epre = (u32)(__shfl(pc, tj) + seg_scan_incl<SEG_GPT>(ent) - ent);
     if (g < ngroups) {
-      const uint4* q = reinterpret_cast<const uint4*>(src + (i64)g * 32);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint4 x = q[j];
-        wv[4 * j] = x.x;
-        wv[4 * j + 1] = x.y;
-        wv[4 * j + 2] = x.z;
-        wv[4 * j + 3] = x.w;
-      }
-#pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        mask |= (wv[j] != 0u ? 1u : 0u) << j;
-        ent += __popc(wv[j]);
-      }
-    }
-    const u64 mine = ((u64)__popc(mask) << 32) | (u32)ent;
-    const u64 ex = seg_scan_incl<SEG_GPT>(mine) - mine;
-    const int tj = (g / SEG_GPT) & (WAVE - 1);  // the group's tile in the window (< 64)
-    const int tpw = __shfl(pw, tj), tpc = __shfl(pc, tj);
-    if (g < ngroups) {
-      int slot = tpw + (int)(ex >> 32);
-      u32 epre = (u32)tpc + (u32)ex;
-      s_gt[g] = ((u64)(u32)slot << 32) | mask;
-#pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        if (wv[j] != 0u) {
-          s_cwd[slot++] = ((u64)epre << 47) | ((u64)(u32)(g * 32 + j) << 32) | wv[j];
-          epre += __popc(wv[j]);
-        }
+      s_gt[g] = ((u64)(u32)slot0 << 32) | mask;
+      u32 mm = mask;
+      for (int slot = slot0; mm; ++slot) {
+        const int j = __ffs(mm) - 1;
+        mm &= mm - 1;
+        const u32 wv = s_r[slot];
+        s_cwd[slot] = ((u64)epre << 47) | ((u64)(u32)(g * 32 + j) << 32) | wv;
+        epre += __popc(wv);
       }
     }
   }
@@ -1425,13 +1474,14 @@ __device__ __forceinline__ void stage_words(const u32* __restrict__ src, const i
 // out..out+n-1 of C with coalesced stores: the values first; then the
 // threads turn the accumulators of the staged words (slot l, l + WG, ...)
 // into column offsets in place; these are written and the array re-zeroed.
-// Three barriers.
+// C is written once and never read here: non-temporal stores keep it from
+// evicting B's rows from L2.  Three barriers.
 template <int WG>
 __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n, i64 out,
                                             i64 cbase, float* acc, i32* __restrict__ ccol,
                                             float* __restrict__ cval) {
   const int l = threadIdx.x;
-  for (int q = l; q < n; q += WG) cval[out + q] = acc[q];
+  for (int q = l; q < n; q += WG) __builtin_nontemporal_store(acc[q], &cval[out + q]);
   __syncthreads();
   i32* colx = reinterpret_cast<i32*>(acc);
   for (int slot = l; slot < nwords; slot += WG) {
@@ -1446,7 +1496,7 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
   }
   __syncthreads();
   for (int q = l; q < n; q += WG) {
-    ccol[out + q] = (i32)(cbase + colx[q]);
+    __builtin_nontemporal_store((i32)(cbase + colx[q]), &ccol[out + q]);
     colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
   }
   __syncthreads();
@@ -1456,8 +1506,8 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
 template <bool BATCH, class OffT, int HC, int ROWS>
 __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
-    const i64* __restrict__ key, i64 th, const i64* __restrict__ arp,
-    const i32* __restrict__ acol, const float* __restrict__ aval,
+    const u32* __restrict__ rowmask, const i64* __restrict__ key, i64 th,
+    const i64* __restrict__ arp, const i32* __restrict__ acol, const float* __restrict__ aval,
     const i64* __restrict__ brp, const uint2* __restrict__ bcv, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
     float* __restrict__ cval, i32* __restrict__ next) {
@@ -1493,7 +1543,9 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     for (int t = l; t < min(nt, TC_LDS); t += WG) s_tc[t] = tc[t];
     const i64 a0 = arp[row], a1 = arp[row + 1], nA = a1 - a0;
     const i64 bb = BATCH ? bbase[row] : 0;
-    const u32* bits = rowbits + h * row_words;
+    const u32* bits = rowbits + h * row_words;                 // packed nonzero words
+    const u32* masks = rowmask + h * ((i64)nt * SEG_GPT);     // 16 group masks per tile
+    i64 wbase = 0;  // packed words of the row's tiles before the current step
     i64 out = crp[row];
     const bool inreg = nA <= (i64)HC * WG;
     const int nch = (int)((nA + WG - 1) / WG);
@@ -1546,17 +1598,28 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
       u64* s_gt = s_cwd + P.w;
       u32 dword = 0;  // dense: this thread's word of the tile
       if (P.dense) {
-        dword = bits[(i64)t * (TILE_W / 32) + l];
         if (dz1 > dz0) {
           for (int d = dz0 + l; d < dz1; d += WG) s_acc[d] = 0.f;
           dz0 = dz1 = 0;
-          __syncthreads();
         }
+        // This thread's word of the tile, from the compressed bitmap: the
+        // tile's group masks (in s_ex, free between chunks) locate it among
+        // the packed words.
+        if (l < SEG_GPT) s_ex[l] = (i32)masks[(i64)t * SEG_GPT + l];
+        __syncthreads();
+        const int g = l >> 5, j = l & 31;
+        int pre = 0;
+        for (int q = 0; q < g; ++q) pre += __popc((u32)s_ex[q]);
+        const u32 m = (u32)s_ex[g];
+        dword = (m >> j) & 1u ? bits[wbase + pre + __popc(m & ((1u << j) - 1u))] : 0u;
       } else {
-        // Accumulators [0, n) start at zero: clear what staged data they
-        // overlap (ordered before the products by the staging barrier).
-        for (int d = dz0 + l; d < min(dz1, n); d += WG) s_acc[d] = 0.f;
-        stage_words<WG>(bits + (i64)t * (TILE_W / 32), s_tc, tc, nt, t, t1, s_cwd, s_gt);
+        stage_packed<WG>(bits + wbase, masks + (i64)t * SEG_GPT, s_tc, tc, nt, t, t1, P.w,
+                         reinterpret_cast<u32*>(s_raw), s_cwd, s_gt);
+        // Accumulators [0, n) start at zero: clear the copied words and the
+        // earlier staged data they overlap (ordered before the products by
+        // the chunk's scan barriers).
+        for (int d = l; d < P.w; d += WG) s_acc[d] = 0.f;
+        for (int d = max(dz0, P.w) + l; d < min(dz1, n); d += WG) s_acc[d] = 0.f;
         const int end = (int)((reinterpret_cast<unsigned char*>(s_gt + (t1 - t) * SEG_GPT) -
                                s_raw) / 4);
         dz0 = n;  // the output re-zeroes [0, n)
@@ -1570,7 +1633,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         int x[ROWS];
         u32 old[ROWS];
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) x[r] = ok[r] ? (int)(c[r] - cbase) : 0;
+        for (int r = 0; r < ROWS; ++r) x[r] = (int)(c[r] - cbase);  // in the tile for every lane
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) old[r] = __float_as_uint(s_acc[x[r]]);
         lds_add_f32_rows<ROWS>(s_acc, x, p, ok, old);
@@ -1589,7 +1652,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
           for (int r = 0; r < H; ++r) {
             okh[r] = ok[h0 + r];
             ph[r] = p[h0 + r];
-            x[r] = okh[r] ? (int)(c[h0 + r] - cbase) : 0;
+            x[r] = (int)(c[h0 + r] - cbase);  // a column of the step for every lane
           }
 #pragma unroll
           for (int r = 0; r < H; ++r) gg[r] = s_gt[x[r] >> 10];  // first slot << 32 | nonzero-word mask
@@ -1601,8 +1664,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
           }
 #pragma unroll
           for (int r = 0; r < H; ++r)
-            rank[r] = okh[r] ? (int)(e[r] >> 47) + __popc((u32)e[r] & ((1u << (x[r] & 31)) - 1u))
-                             : 0;
+            rank[r] = (int)(e[r] >> 47) + __popc((u32)e[r] & ((1u << (x[r] & 31)) - 1u));
 #pragma unroll
           for (int r = 0; r < H; ++r) old[r] = __float_as_uint(s_acc[rank[r]]);
           lds_add_f32_rows<H>(s_acc, rank, ph, okh, old);
@@ -1664,8 +1726,8 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         while (r) {
           const int x = l * 32 + (__ffs(r) - 1);
           r &= r - 1;
-          ccol[q] = (i32)(cbase + x);
-          cval[q] = s_acc[x];
+          __builtin_nontemporal_store((i32)(cbase + x), &ccol[q]);
+          __builtin_nontemporal_store(s_acc[x], &cval[q]);
           s_acc[x] = 0.f;
           ++q;
         }
@@ -1674,6 +1736,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         emit_ranked<WG>(s_cwd, P.w, n, out, cbase, s_acc, ccol, cval);
       }
       out += n;
+      wbase += P.w;
       P = Q;
     }
     __syncthreads();  // s_tc is reloaded for the next row
@@ -1865,13 +1928,15 @@ void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const 
 template <bool BATCH>
 void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr& B,
                        const i64* bbase, const i32* slab, i32 nt, i32* tilecnt, i64* nnz,
-                       u32* rowbits, i32* next, hipStream_t s) {
+                       u32* rowbits, u32* rowmask, i32* next, hipStream_t s) {
   if (B.nnz < ((i64)1 << 31))
     hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8>), dim3(grid), dim3(HWG), 0, s, rs, A.rp,
-                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, next);
+                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, rowmask,
+                       next);
   else
     hipLaunchKernelGGL((k_sym_bitmap<BATCH, i64, 4>), dim3(grid), dim3(HWG), 0, s, rs, A.rp,
-                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, next);
+                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, rowmask,
+                       next);
 }
 
 HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
@@ -1903,11 +1968,15 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     heavy.tilecnt.alloc((size_t)heavy.n * heavy.nt);
     // Keep the long rows' column bitmaps for the segmented numeric kernel
     // when they fit comfortably in free device memory.
+    // (Worst case: every word nonzero.)
     const size_t bm_bytes = (size_t)heavy.n * heavy.nt * (TILE_W / 8);
+    const size_t mask_bytes = (size_t)heavy.n * heavy.nt * (TILE_W / 32 / 32) * sizeof(u32);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    if (bm_bytes <= free_b / 3 + pool_cached_bytes() / 3)
+    if (bm_bytes + mask_bytes <= free_b / 3 + pool_cached_bytes() / 3) {
       heavy.rowbits.alloc(bm_bytes / sizeof(u32));
+      heavy.rowmask.alloc(mask_bytes / sizeof(u32));
+    }
     // Tile boundaries of every B row (kept for the numeric pass).
     heavy.slab.alloc((size_t)B.rows * (heavy.nt + 1));
     if (B.rows > 0) {
@@ -1923,7 +1992,7 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     ProfScope ps("sym_bitmap", s);
     const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
     launch_sym_bitmap<BATCH>(grid, rs, A, B, bbase, heavy.slab.p, heavy.nt, heavy.tilecnt.p, nnz,
-                             heavy.rowbits.p, heavy.next.p, s);
+                             heavy.rowbits.p, heavy.rowmask.p, heavy.next.p, s);
     RT_HIP(hipGetLastError());
   }
 }
@@ -1957,11 +2026,13 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
       // entries are indexable by int32.
       if (heavy.rowbits.p && B.nnz < ((i64)1 << 31))
         hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i32, 1, 8>), dim3(grid), dim3(HWG), 0, s, rs,
-                           heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
+                           heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, heavy.rowmask.p, key,
+                           tu.num[4], A.rp, A.col,
                            A.val, B.rp, pairs.p, bbase, slab, C.rp, C.col, C.val, next.p);
       else if (heavy.rowbits.p)
         hipLaunchKernelGGL((k_num_heavy_seg<BATCH, i64, 2, 4>), dim3(grid), dim3(HWG), 0, s, rs,
-                           heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, key, tu.num[4], A.rp, A.col,
+                           heavy.nt, heavy.tilecnt.p, heavy.rowbits.p, heavy.rowmask.p, key,
+                           tu.num[4], A.rp, A.col,
                            A.val, B.rp, pairs.p, bbase, slab, C.rp, C.col, C.val, next.p);
       else
         hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(grid), dim3(HWG), 0, s, rs,
@@ -2303,7 +2374,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p, s);
       launch_sym_bitmap<BATCH>(heavy_grid * 2, heavy, A, B, bbase, slab.p, nt, tilecnt.p, nnz.p,
-                               nullptr, next.p, s);
+                               nullptr, nullptr, next.p, s);
       hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
       hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nbs, C.rp + m);
       if (m > 0)

@@ -54,7 +54,11 @@ struct HeavyRows {
   DBuf<i32> list;  // all binned rows; the heavy ones start at `off`
   i32 off = 0, n = 0, nt = 1;
   DBuf<i32> tilecnt;  // n x nt: entries | nonzero bitmap words << 16 per tile
-  DBuf<u32> rowbits;  // n x nt x TILE_W/32 words: each long row's column set (may be empty)
+  // Each long row's column set (both may be empty): per tile 16 masks of
+  // its nonzero 32-column words (rowmask, n x nt x 16), and those words
+  // packed in column order (rowbits, n x nt x TILE_W/32 words of room).
+  DBuf<u32> rowbits;
+  DBuf<u32> rowmask;
   DBuf<i32> next;     // row-slot counter of the symbolic long-row kernel
   DBuf<i32> slab;     // B.rows x (nt+1) tile boundaries of B's rows
 };
