@@ -613,6 +613,12 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   constexpr int WPL = BMW / G;  // bitmap words per lane
   static_assert(BMW % G == 0, "bitmap words split evenly over the group");
   __shared__ u32 s_bm[NUMERIC && !BM_IN_ENT ? R * T : 1];
+  // Per bitmap word, the count of set bits before it (the rank of its first
+  // column in the row).  A whole-workgroup row whose bitmap is not in s_ent
+  // keeps them there (idle once the products are in the table) when they fit.
+  constexpr bool WP_IN_ENT = G == WG && !BM_IN_ENT && 2 * BMW <= 16 * WG;
+  __shared__ u16 s_wpre[NUMERIC && !WP_IN_ENT ? R * BMW : 1];
+  static_assert(T <= 65535, "u16 word prefixes (a row has fewer than T entries)");
   const i32 nlist = a.dcount ? a.dcount[a.bin] : a.nlist;
   const i32* __restrict__ list = a.dcount ? a.list + a.doff[a.bin] : a.list;
   const i64* __restrict__ arp = a.arp;
@@ -778,9 +784,11 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
       }
     }
     // Column range of the row.  When it is narrower than the bitmap, the
-    // row's columns are set in the bitmap and lane l writes the entries of
-    // words [l*WPL, (l+1)*WPL) in column order at their rank (prefix of
-    // set bits), reading each value back from the table: no sort.
+    // row's columns are set in the bitmap, each word's prefix of set bits is
+    // stored, and every occupied slot finds its column's rank in the row
+    // (word prefix + set bits below it in the word): the entries are staged
+    // in column order in the key array (packed column << 32 | value bits)
+    // and written with coalesced stores.  No sort, no second probe.
     i32 cmn = INT_MAX, cmx = -1;
 #pragma unroll
     for (int m = 0; m < SPL; ++m) {
@@ -804,13 +812,10 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
         }
       }
       group_sync<G>();
-      u32 wv[WPL];
+      u16* wpre = WP_IN_ENT ? reinterpret_cast<u16*>(s_ent) : s_wpre + g * BMW;
       int cnt = 0;
 #pragma unroll
-      for (int j = 0; j < WPL; ++j) {
-        wv[j] = bm[l * WPL + j];
-        cnt += __popc(wv[j]);
-      }
+      for (int j = 0; j < WPL; ++j) cnt += __popc(bm[l * WPL + j]);
       int inc, N;
       if constexpr (G <= WAVE) {
         inc = seg_scan_incl<G>(cnt);
@@ -818,21 +823,41 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
       } else {
         inc = block_scan_incl<WG>(cnt, s_scr, &N);
       }
-      if (active) {
-        const i64 off = a.crp[row];
-        i64 q = off + (inc - cnt);
+      {
+        int run = inc - cnt;
 #pragma unroll
         for (int j = 0; j < WPL; ++j) {
-          u32 bits = wv[j];
-          while (bits) {
-            const i32 c = cmn + (l * WPL + j) * 32 + (__ffs(bits) - 1);
-            bits &= bits - 1;
-            u32 h = hash_col<LT>(c);
-            while (keys[h] != c) h = (h + 1) & (T - 1);
-            a.ccol[q] = c;
-            a.cval[q] = vals[h];
-            ++q;
-          }
+          wpre[l * WPL + j] = (u16)run;
+          run += __popc(bm[l * WPL + j]);
+        }
+      }
+      group_sync<G>();
+      u64 ent[SPL];
+      int rk[SPL];
+#pragma unroll
+      for (int m = 0; m < SPL; ++m) {
+        const int s = l + m * G;
+        const i32 kk = keys[s];
+        rk[m] = -1;
+        ent[m] = 0;
+        if (kk != EMPTY_KEY) {
+          const int x = kk - cmn;
+          rk[m] = wpre[x >> 5] + __popc(bm[x >> 5] & ((1u << (x & 31)) - 1u));
+          ent[m] = ((u64)(u32)kk << 32) | __float_as_uint(vals[s]);
+        }
+      }
+      group_sync<G>();  // every slot is read before the staging overwrites the keys
+      u64* buf = reinterpret_cast<u64*>(keys);  // T/2 packed entries; N <= T/2
+#pragma unroll
+      for (int m = 0; m < SPL; ++m)
+        if (rk[m] >= 0) buf[rk[m]] = ent[m];
+      group_sync<G>();
+      if (active) {
+        const i64 off = a.crp[row];
+        for (int q = l; q < N; q += G) {
+          const u64 v = buf[q];
+          a.ccol[off + q] = (i32)(v >> 32);
+          a.cval[off + q] = __uint_as_float((u32)v);
         }
         if constexpr (MODE == MODE_LEARN) {
           if (l == 0) a.nnz[row] = N;
