@@ -605,19 +605,14 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   float* s_av = reinterpret_cast<float*>(s_ent + 12 * WG);
   __shared__ i32 s_scr[WG / WAVE];
   __shared__ i32 s_ovf[MODE == MODE_LEARN ? R : 1];
-  // Column bitmap of a numeric row (BMW words per row): the output order
-  // when the row's columns span fewer than 32*BMW.  Whole-workgroup rows of
-  // a large workgroup reuse s_ent, idle once the products are in the table.
-  constexpr bool BM_IN_ENT = G == WG && 4 * WG >= 2048;  // 65536 columns or more
-  constexpr int BMW = BM_IN_ENT ? 4 * WG : T;
+  // Column bitmap of a numeric row: the output order when the row's columns
+  // span fewer than 32*T.  Once every lane holds its slots in registers the
+  // table is free: the bitmap takes the key array (T words), the words'
+  // prefixes of set bits the value array (u16), and the row is staged in
+  // column order in the key array again.
+  constexpr int BMW = T;
   constexpr int WPL = BMW / G;  // bitmap words per lane
   static_assert(BMW % G == 0, "bitmap words split evenly over the group");
-  __shared__ u32 s_bm[NUMERIC && !BM_IN_ENT ? R * T : 1];
-  // Per bitmap word, the count of set bits before it (the rank of its first
-  // column in the row).  A whole-workgroup row whose bitmap is not in s_ent
-  // keeps them there (idle once the products are in the table) when they fit.
-  constexpr bool WP_IN_ENT = G == WG && !BM_IN_ENT && 2 * BMW <= 16 * WG;
-  __shared__ u16 s_wpre[NUMERIC && !WP_IN_ENT ? R * BMW : 1];
   static_assert(T <= 65535, "u16 word prefixes (a row has fewer than T entries)");
   const i32 nlist = a.dcount ? a.dcount[a.bin] : a.nlist;
   const i32* __restrict__ list = a.dcount ? a.list + a.doff[a.bin] : a.list;
@@ -783,36 +778,43 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
         return;
       }
     }
-    // Column range of the row.  When it is narrower than the bitmap, the
-    // row's columns are set in the bitmap, each word's prefix of set bits is
-    // stored, and every occupied slot finds its column's rank in the row
-    // (word prefix + set bits below it in the word): the entries are staged
-    // in column order in the key array (packed column << 32 | value bits)
-    // and written with coalesced stores.  No sort, no second probe.
+    // Every lane takes its slots (packed column << 32 | value bits; ~0 when
+    // empty) and the row's column range.  When the range is narrower than
+    // the bitmap, the row's columns are set in the bitmap, each word's
+    // prefix of set bits is stored, and every entry finds its rank in the
+    // row (word prefix + set bits below it in the word): the row is staged
+    // in column order and written with coalesced stores.  No sort, no second
+    // probe.  Wider rows are compacted and sorted.
+    u64 ent[SPL];
     i32 cmn = INT_MAX, cmx = -1;
+    int mc = 0;
 #pragma unroll
     for (int m = 0; m < SPL; ++m) {
-      const i32 kk = keys[l + m * G];
+      const int s = l + m * G;
+      const i32 kk = keys[s];
+      ent[m] = ~0ull;
       if (kk != EMPTY_KEY) {
+        ent[m] = ((u64)(u32)kk << 32) | __float_as_uint(vals[s]);
         cmn = min(cmn, kk);
         cmx = max(cmx, kk);
+        ++mc;
       }
     }
     group_minmax<G, WG>(cmn, cmx, s_scr);
+    group_sync<G>();  // every slot is read before the table is reused
     if (cmx >= 0 && (i64)cmx - cmn < (i64)BMW * 32) {  // uniform over the group
-      u32* bm = BM_IN_ENT ? reinterpret_cast<u32*>(s_ent) : s_bm + (BM_IN_ENT ? 0 : g * T);
+      u32* bm = reinterpret_cast<u32*>(keys);
+      u16* wpre = reinterpret_cast<u16*>(vals);
       for (int d = l; d < BMW; d += G) bm[d] = 0u;
       group_sync<G>();
 #pragma unroll
       for (int m = 0; m < SPL; ++m) {
-        const i32 kk = keys[l + m * G];
-        if (kk != EMPTY_KEY) {
-          const int x = kk - cmn;
+        if (ent[m] != ~0ull) {
+          const int x = (i32)(ent[m] >> 32) - cmn;
           lds_or_u32(&bm[x >> 5], 1u << (x & 31));
         }
       }
       group_sync<G>();
-      u16* wpre = WP_IN_ENT ? reinterpret_cast<u16*>(s_ent) : s_wpre + g * BMW;
       int cnt = 0;
 #pragma unroll
       for (int j = 0; j < WPL; ++j) cnt += __popc(bm[l * WPL + j]);
@@ -832,21 +834,16 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
         }
       }
       group_sync<G>();
-      u64 ent[SPL];
       int rk[SPL];
 #pragma unroll
       for (int m = 0; m < SPL; ++m) {
-        const int s = l + m * G;
-        const i32 kk = keys[s];
         rk[m] = -1;
-        ent[m] = 0;
-        if (kk != EMPTY_KEY) {
-          const int x = kk - cmn;
+        if (ent[m] != ~0ull) {
+          const int x = (i32)(ent[m] >> 32) - cmn;
           rk[m] = wpre[x >> 5] + __popc(bm[x >> 5] & ((1u << (x & 31)) - 1u));
-          ent[m] = ((u64)(u32)kk << 32) | __float_as_uint(vals[s]);
         }
       }
-      group_sync<G>();  // every slot is read before the staging overwrites the keys
+      group_sync<G>();  // every rank is known before the staging overwrites the bitmap
       u64* buf = reinterpret_cast<u64*>(keys);  // T/2 packed entries; N <= T/2
 #pragma unroll
       for (int m = 0; m < SPL; ++m)
@@ -865,16 +862,7 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
       }
       return;
     }
-    // Otherwise compact the occupied slots (packed column<<32 | value bits) ...
-    u64 mine[SPL];
-    int mc = 0;
-#pragma unroll
-    for (int m = 0; m < SPL; ++m) {
-      const int s = l + m * G;
-      const i32 kk = keys[s];
-      mine[m] = (kk == EMPTY_KEY) ? ~0ull : (((u64)(u32)kk << 32) | __float_as_uint(vals[s]));
-      mc += (kk != EMPTY_KEY);
-    }
+    // Otherwise compact the entries ...
     int inc, N;
     if constexpr (G <= WAVE) {
       inc = seg_scan_incl<G>(mc);
@@ -883,11 +871,10 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
       inc = block_scan_incl<WG>(mc, s_scr, &N);
     }
     int pos = inc - mc;
-    group_sync<G>();
     u64* buf = reinterpret_cast<u64*>(keys);  // T/2 packed entries; N <= T/2
 #pragma unroll
     for (int m = 0; m < SPL; ++m)
-      if (mine[m] != ~0ull) buf[pos++] = mine[m];
+      if (ent[m] != ~0ull) buf[pos++] = ent[m];
     int S = 1;
     while (S < N) S <<= 1;
     for (int q = N + l; q < S; q += G) buf[q] = ~0ull;
