@@ -541,6 +541,47 @@ __device__ __forceinline__ void lds_add_f32_rows(float* base, const int (&x)[ROW
 //               in a.ovf[row] for the exact rerun.  a.nnz[row] gets the count.
 constexpr int MODE_SYM = 0, MODE_NUM = 1, MODE_LEARN = 2;
 
+// One product into a row's LDS hash table (T slots, linear probing), with
+// the key of its first probe slot h already read (kk).  MODE_SYM counts
+// new columns in mycnt; MODE_NUM adds v into the column's value; MODE_LEARN
+// also counts, gives up after a probe run of PLIM slots and flags the row
+// in *ovf instead.
+template <int MODE, int T>
+__device__ __forceinline__ void hash_insert(i32* keys, float* vals, i32 c, u32 h, i32 kk, float v,
+                                            int& mycnt, i32* ovf) {
+  if constexpr (MODE == MODE_LEARN) {
+    if (*ovf) return;
+    constexpr int PLIM = T < 64 ? T : 64;
+    bool found = false;
+    for (int probe = 0; probe < PLIM; ++probe) {
+      if (kk == c) { found = true; break; }
+      if (kk == EMPTY_KEY) {
+        kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+        if (kk == EMPTY_KEY) { ++mycnt; found = true; break; }
+        if (kk == c) { found = true; break; }
+      }
+      h = (h + 1) & (T - 1);
+      kk = keys[h];
+    }
+    if (found) lds_add_f32(&vals[h], v); else *ovf = 1;
+  } else {
+    while (true) {
+      if (kk == c) break;
+      if (kk == EMPTY_KEY) {
+        kk = atomicCAS(&keys[h], EMPTY_KEY, c);
+        if (kk == EMPTY_KEY) {
+          if constexpr (MODE == MODE_SYM) ++mycnt;
+          break;
+        }
+        if (kk == c) break;
+      }
+      h = (h + 1) & (T - 1);
+      kk = keys[h];
+    }
+    if constexpr (MODE != MODE_SYM) lds_add_f32(&vals[h], v);
+  }
+}
+
 struct HashArgs {
   const i32* list;
   i32 nlist;
@@ -590,7 +631,9 @@ __device__ __forceinline__ void group_minmax(i32& mn, i32& mx, i32* scr) {
 }
 
 template <int G, int T, int WG, int MODE, bool BATCH>
-__global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
+// A 512-thread workgroup keeps two resident per CU: four waves per SIMD
+// (the second bound), so at most 128 VGPRs.
+__global__ __launch_bounds__(WG, WG >= 512 ? 4 : 1) void k_hash(HashArgs a) {
   static_assert(G <= WAVE || G == WG, "a multi-wave group must be the whole workgroup");
   constexpr bool NUMERIC = MODE != MODE_SYM;
   constexpr int R = WG / G;
@@ -614,6 +657,13 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   constexpr int WPL = BMW / G;  // bitmap words per lane
   static_assert(BMW % G == 0, "bitmap words split evenly over the group");
   static_assert(T <= 65535, "u16 word prefixes (a row has fewer than T entries)");
+  // Whole-workgroup rows enumerate their products with chunk_products (owner
+  // marks or 64-product blocks, HR products per lane in flight).
+  constexpr bool WHOLE = G == WG;
+  constexpr int HR = MODE == MODE_LEARN ? 2 : 4;  // learned rows: fewer registers
+  __shared__ i32 s_cex[WHOLE ? WG + 1 : 1];
+  __shared__ __align__(16) u16 s_cmark[WHOLE ? HR * WG : 8];
+  __shared__ u64 s_cscr[WHOLE ? WG / WAVE : 1];
   const i32 nlist = a.dcount ? a.dcount[a.bin] : a.nlist;
   const i32* __restrict__ list = a.dcount ? a.list + a.doff[a.bin] : a.list;
   const i64* __restrict__ arp = a.arp;
@@ -629,6 +679,10 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
   float* vals = NUMERIC ? s_vals + g * T : nullptr;
   const int gb = g * G;
   const i64 ngroups = ((i64)nlist + R - 1) / R;
+  if constexpr (WHOLE) {
+    for (int d = threadIdx.x; d < HR * WG; d += WG) s_cmark[d] = 0;  // zeroed between chunks
+    __syncthreads();
+  }
   // Row groups are taken grid-stride (a single pass when the grid covers the
   // bin).  The loop is uniform over the workgroup; in block mode the tables
   // are shared by the whole workgroup, hence the barrier between groups.
@@ -655,6 +709,45 @@ __global__ __launch_bounds__(WG) void k_hash(HashArgs a) {
     if constexpr (MODE == MODE_LEARN) cap = a.cap[row];
   }
   int mycnt = 0;
+  if constexpr (WHOLE) {
+    // Batched insert of HR products per lane: the first probe of every row
+    // is read before any is used.
+    static_assert(HR == 2 || HR == 4, "named first probes");
+    auto ins = [&](const auto& c, const auto& p, const auto& ok) {
+      i32* ovf = MODE == MODE_LEARN ? &s_ovf[g] : nullptr;
+      const u32 h0 = hash_col<LT>(c[0]), h1 = hash_col<LT>(c[1]);
+      if constexpr (HR == 4) {
+        const u32 h2 = hash_col<LT>(c[2]), h3 = hash_col<LT>(c[3]);
+        const i32 q0 = keys[h0], q1 = keys[h1], q2 = keys[h2], q3 = keys[h3];
+        if (ok[0]) hash_insert<MODE, T>(keys, vals, c[0], h0, q0, p[0], mycnt, ovf);
+        if (ok[1]) hash_insert<MODE, T>(keys, vals, c[1], h1, q1, p[1], mycnt, ovf);
+        if (ok[2]) hash_insert<MODE, T>(keys, vals, c[2], h2, q2, p[2], mycnt, ovf);
+        if (ok[3]) hash_insert<MODE, T>(keys, vals, c[3], h3, q3, p[3], mycnt, ovf);
+      } else {
+        const i32 q0 = keys[h0], q1 = keys[h1];
+        if (ok[0]) hash_insert<MODE, T>(keys, vals, c[0], h0, q0, p[0], mycnt, ovf);
+        if (ok[1]) hash_insert<MODE, T>(keys, vals, c[1], h1, q1, p[1], mycnt, ovf);
+      }
+    };
+    const ChunkLds<i64> cl{s_bs, s_av, s_cex, s_cmark, s_cscr};
+    for (i64 e0 = a0; e0 < a1; e0 += WG) {  // uniform: one row per workgroup
+      const i64 e = e0 + l;
+      int len = 0;
+      i64 bs = 0;
+      float av = 0.f;
+      if (e < a1) {
+        const i64 k = (i64)acol[e] + bb;
+        bs = brp[k];
+        len = (int)(brp[k + 1] - bs);
+        if constexpr (NUMERIC) av = aval[e];
+      }
+      const int nvalid = (int)min((i64)WG, a1 - e0);
+      if constexpr (NUMERIC)
+        chunk_products<WG, HR, true>(len, bs, av, nvalid, SplitEntries{bcol, bval}, cl, ins);
+      else
+        chunk_products<WG, HR, false>(len, bs, 0.f, nvalid, ColumnEntries{bcol}, cl, ins);
+    }
+  } else
   for (i64 e0 = a0; e0 < a1; e0 += G) {
     const i64 e = e0 + l;
     int len = 0;
