@@ -56,12 +56,16 @@ __global__ __launch_bounds__(256) void k_features(i64 m, const i64* __restrict__
                                                   const i64* __restrict__ brp,
                                                   const i32* __restrict__ bcol,
                                                   const i64* __restrict__ bbase, i64 ncols,
-                                                  const i64* __restrict__ prod,
+                                                  const i64* __restrict__ prod, i64 skip_u,
                                                   float* __restrict__ feat) {
   constexpr int G = 16;
   const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / G;
   const int l = threadIdx.x & (G - 1);
   if (row >= m) return;  // whole group leaves together
+  if (min(prod[row], ncols) > skip_u) {  // a row the caller sends to the exact path unseen
+    if (l < NF) feat[row * NF + l] = 0.f;
+    return;
+  }
   const i64 a0 = arp[row], a1 = arp[row + 1], nA = a1 - a0;
   const i64 bb = BATCH ? bbase[row] : 0;
   i64 maxlen = 0;
@@ -422,17 +426,19 @@ void upload(DBuf<T>& d, const T* h, size_t n, hipStream_t s) {
   if (n) RT_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// Rows whose product bound min(P_i, cols) exceeds skip_u get zero features
+// (their prediction is not used).
 void features_dev(const Csr& A, const Csr& B, const i64* bbase, const i64* prod, float* feat,
-                  hipStream_t s) {
+                  hipStream_t s, i64 skip_u = LLONG_MAX) {
   if (A.rows == 0) return;
   const dim3 grid((unsigned)((A.rows * 16 + 255) / 256));
   ProfScope ps("row_features", s);
   if (bbase)
     hipLaunchKernelGGL(k_features<true>, grid, dim3(256), 0, s, A.rows, A.rp, A.col, B.rp, B.col,
-                       bbase, B.cols, prod, feat);
+                       bbase, B.cols, prod, skip_u, feat);
   else
     hipLaunchKernelGGL(k_features<false>, grid, dim3(256), 0, s, A.rows, A.rp, A.col, B.rp, B.col,
-                       nullptr, B.cols, prod, feat);
+                       nullptr, B.cols, prod, skip_u, feat);
   RT_HIP(hipGetLastError());
 }
 
@@ -473,7 +479,10 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   RunStats& S = st ? *st : local;
   row_products(A, B, bbase, prod.p, s, &S);
   RT_HIP(hipMemsetAsync(nnz.p, 0, M * sizeof(i64), s));
-  features_dev(A, B, bbase, prod.p, feat.p, s);
+  // Long rows go to the exact path unpredicted unless the learned long-row
+  // kernel is chosen (the bound k_capacity applies).
+  const i64 long_th = std::min(tu.sym[4], tu.num[4]);
+  features_dev(A, B, bbase, prod.p, feat.p, s, tu.learned_heavy ? LLONG_MAX : long_th);
   predict_dev(params.p, feat.p, m, pred.p, s);
   const dim3 g256((unsigned)((M + 255) / 256));
   {
