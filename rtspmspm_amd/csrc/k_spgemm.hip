@@ -1141,7 +1141,11 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
           if (gw[j] != 0u) s_bits[k++] = gw[j];
         __syncthreads();
         u32* dst = rowbits + it * ((i64)nt * (TILE_W / 32)) + wcarry;
-        for (int q = l; q < wtot; q += WG) __builtin_nontemporal_store(s_bits[q], &dst[q]);
+        // Lanes shifted to 128-byte line boundaries (non-temporal stores are
+        // not merged in L2).
+        const int q0 = l - (int)(reinterpret_cast<uintptr_t>(dst) / 4 & 31);
+        for (int q = q0; q < wtot; q += WG)
+          if (q >= 0) __builtin_nontemporal_store(s_bits[q], &dst[q]);
         wcarry += wtot;
       }
       // (ends with a barrier: s_bits is read before the next window clears it)
@@ -1580,13 +1584,18 @@ __device__ __forceinline__ void stage_packed(const u32* __restrict__ pw,
 // threads turn the accumulators of the staged words (slot l, l + WG, ...)
 // into column offsets in place; these are written and the array re-zeroed.
 // C is written once and never read here: non-temporal stores keep it from
-// evicting B's rows from L2.  Three barriers.
+// evicting B's rows from L2.  Such stores are not merged in L2, so the lanes
+// are shifted to line boundaries (32 entries): every wave then writes whole
+// 128-byte lines, and only a step's two end lines are partial.  Three
+// barriers.
 template <int WG>
 __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n, i64 out,
                                             i64 cbase, float* acc, i32* __restrict__ ccol,
                                             float* __restrict__ cval) {
   const int l = threadIdx.x;
-  for (int q = l; q < n; q += WG) __builtin_nontemporal_store(acc[q], &cval[out + q]);
+  const int q0 = l - (int)(out & 31);
+  for (int q = q0; q < n; q += WG)
+    if (q >= 0) __builtin_nontemporal_store(acc[q], &cval[out + q]);
   __syncthreads();
   i32* colx = reinterpret_cast<i32*>(acc);
   for (int slot = l; slot < nwords; slot += WG) {
@@ -1600,9 +1609,11 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
     }
   }
   __syncthreads();
-  for (int q = l; q < n; q += WG) {
-    __builtin_nontemporal_store((i32)(cbase + colx[q]), &ccol[out + q]);
-    colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
+  for (int q = q0; q < n; q += WG) {
+    if (q >= 0) {
+      __builtin_nontemporal_store((i32)(cbase + colx[q]), &ccol[out + q]);
+      colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
+    }
   }
   __syncthreads();
 }
