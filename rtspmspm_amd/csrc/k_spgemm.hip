@@ -691,15 +691,8 @@ __global__ __launch_bounds__(WG, WG >= 512 ? 4 : 1) void k_hash(HashArgs a) {
   const i64 ridx = gi * R + g;
   const bool active = ridx < nlist;
 
-  for (int s = l; s < T; s += G) {
-    keys[s] = EMPTY_KEY;
-    if constexpr (NUMERIC) vals[s] = 0.f;
-  }
-  if constexpr (MODE == MODE_LEARN) {
-    if (l == 0) s_ovf[g] = 0;
-  }
-  group_sync<G>();
-
+  // The row's header loads are issued first; the table is cleared while
+  // they are in flight.
   const i32 row = active ? list[ridx] : 0;
   i64 a0 = 0, a1 = 0, bb = 0, cap = 0;
   if (active) {
@@ -708,6 +701,14 @@ __global__ __launch_bounds__(WG, WG >= 512 ? 4 : 1) void k_hash(HashArgs a) {
     if constexpr (BATCH) bb = a.bbase[row];
     if constexpr (MODE == MODE_LEARN) cap = a.cap[row];
   }
+  for (int s = l; s < T; s += G) {
+    keys[s] = EMPTY_KEY;
+    if constexpr (NUMERIC) vals[s] = 0.f;
+  }
+  if constexpr (MODE == MODE_LEARN) {
+    if (l == 0) s_ovf[g] = 0;
+  }
+  group_sync<G>();
   int mycnt = 0;
   if constexpr (WHOLE) {
     // Batched insert of HR products per lane: the first probe of every row
