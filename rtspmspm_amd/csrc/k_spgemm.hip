@@ -545,7 +545,8 @@ constexpr int MODE_SYM = 0, MODE_NUM = 1, MODE_LEARN = 2;
 // the key of its first probe slot h already read (kk).  MODE_SYM counts
 // new columns in mycnt; MODE_NUM adds v into the column's value; MODE_LEARN
 // also counts, gives up after a probe run of PLIM slots and flags the row
-// in *ovf instead.
+// in *ovf instead (so that no lane can spin on a crowded table; the row's
+// distinct count is checked against its predicted capacity afterwards).
 template <int MODE, int T>
 __device__ __forceinline__ void hash_insert(i32* keys, float* vals, i32 c, u32 h, i32 kk, float v,
                                             int& mycnt, i32* ovf) {
@@ -803,51 +804,9 @@ __global__ __launch_bounds__(WG, WG >= 512 ? 4 : 1) void k_hash(HashArgs a) {
       for (int u = 0; u < HU; ++u) {
         if (!okp[u]) continue;
         const i32 c = cu[u];
-        u32 h = hash_col<LT>(c);
-        if constexpr (MODE == MODE_LEARN) {
-          // Probe-limited insert: a row whose table gets crowded (a probe
-          // run of PLIM slots) is abandoned and flagged for the exact rerun,
-          // so no lane can spin; its distinct count is checked against the
-          // predicted capacity once the products are in.
-          if (s_ovf[g]) continue;
-          constexpr int PLIM = T < 64 ? T : 64;
-          const float v = avu[u] * bvu[u];
-          bool ok = false;
-          for (int probe = 0; probe < PLIM; ++probe) {
-            i32 kk = keys[h];
-            if (kk == c) { ok = true; break; }
-            if (kk == EMPTY_KEY) {
-              kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-              if (kk == EMPTY_KEY) { ++mycnt; ok = true; break; }
-              if (kk == c) { ok = true; break; }
-            }
-            h = (h + 1) & (T - 1);
-          }
-          if (ok) lds_add_f32(&vals[h], v); else s_ovf[g] = 1;
-        } else if constexpr (NUMERIC) {
-          const float v = avu[u] * bvu[u];
-          while (true) {
-            i32 kk = keys[h];
-            if (kk == c) break;
-            if (kk == EMPTY_KEY) {
-              kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-              if (kk == EMPTY_KEY || kk == c) break;
-            }
-            h = (h + 1) & (T - 1);
-          }
-          lds_add_f32(&vals[h], v);
-        } else {
-          while (true) {
-            i32 kk = keys[h];
-            if (kk == c) break;
-            if (kk == EMPTY_KEY) {
-              kk = atomicCAS(&keys[h], EMPTY_KEY, c);
-              if (kk == EMPTY_KEY) { ++mycnt; break; }
-              if (kk == c) break;
-            }
-            h = (h + 1) & (T - 1);
-          }
-        }
+        const u32 h = hash_col<LT>(c);
+        hash_insert<MODE, T>(keys, vals, c, h, keys[h], NUMERIC ? avu[u] * bvu[u] : 0.f, mycnt,
+                             MODE == MODE_LEARN ? &s_ovf[g] : nullptr);
       }
     }
     group_sync<G>();
