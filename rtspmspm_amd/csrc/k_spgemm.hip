@@ -1802,8 +1802,11 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         while (r) {
           const int x = l * 32 + (__ffs(r) - 1);
           r &= r - 1;
-          __builtin_nontemporal_store((i32)(cbase + x), &ccol[q]);
-          __builtin_nontemporal_store(s_acc[x], &cval[q]);
+          // Ordinary stores: each lane writes its own short run, and L2
+          // merges the runs into whole lines (non-temporal stores would
+          // reach memory piecemeal).
+          ccol[q] = (i32)(cbase + x);
+          cval[q] = s_acc[x];
           s_acc[x] = 0.f;
           ++q;
         }
