@@ -1998,7 +1998,7 @@ void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const 
   launch_hash<8, 32, 256, MODE, BATCH>(names[MODE][0], a, list + off[0], cnt[0], s);
   launch_hash<16, 128, 256, MODE, BATCH>(names[MODE][1], a, list + off[1], cnt[1], s);
   launch_hash<64, 512, 256, MODE, BATCH>(names[MODE][2], a, list + off[2], cnt[2], s);
-  launch_hash<256, 2048, 256, MODE, BATCH>(names[MODE][3], a, list + off[3], cnt[3], s);
+  launch_hash<128, 2048, 128, MODE, BATCH>(names[MODE][3], a, list + off[3], cnt[3], s);
   launch_hash<512, 8192, 512, MODE, BATCH>(names[MODE][4], a, list + off[4], cnt[4], s);
 }
 
@@ -2373,7 +2373,7 @@ void launch_hash_bins_dev(const HashArgs& a, const DevBins& b, i64 max_rows, hip
   launch_hash_dev<8, 32, 256, MODE, BATCH>(a, b, 0, max_rows, s);
   launch_hash_dev<16, 128, 256, MODE, BATCH>(a, b, 1, max_rows, s);
   launch_hash_dev<64, 512, 256, MODE, BATCH>(a, b, 2, max_rows, s);
-  launch_hash_dev<256, 2048, 256, MODE, BATCH>(a, b, 3, max_rows, s);
+  launch_hash_dev<128, 2048, 128, MODE, BATCH>(a, b, 3, max_rows, s);
   launch_hash_dev<512, 8192, 512, MODE, BATCH>(a, b, 4, max_rows, s);
 }
 
