@@ -156,6 +156,42 @@ def test_long_rows_window_span_limit():
     check_parity(C, oracle.spgemm_f64(A, B))
 
 
+def _mixed_run_operands(seed):
+    """B rows of 1, 15-17, 63-65, 127-129, 500 and 2000 entries over 300,000
+    columns; A rows of 3, 8, 30 or 200 entries picking B rows at random."""
+    rng = np.random.default_rng(seed)
+    ncols, nb, na = 300_000, 600, 400
+    lens = [1, 15, 16, 17, 63, 64, 65, 127, 128, 129, 500, 2000]
+    br, bc = [], []
+    for k in range(nb):
+        cols = np.sort(rng.choice(ncols, lens[k % len(lens)], replace=False))
+        br.append(np.full(cols.size, k))
+        bc.append(cols)
+    br, bc = np.concatenate(br), np.concatenate(bc)
+    B = csr_from_coo(nb, ncols, br, bc, rng.uniform(-1, 1, br.size))
+    ar, ac = [], []
+    for i in range(na):
+        ks = rng.choice(nb, [3, 8, 30, 200][i % 4], replace=False)
+        ar.append(np.full(ks.size, i))
+        ac.append(ks)
+    ar, ac = np.concatenate(ar), np.concatenate(ac)
+    A = csr_from_coo(na, nb, ar, ac, rng.uniform(-1, 1, ar.size))
+    return A, B
+
+
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_chunk_walk_block_and_mark_modes(mode):
+    """Runs that fill 64-product blocks exactly, nearly, or leave them mostly
+    idle: chunks fall on both sides of the blocks-or-owner-marks choice, in
+    the hash kernels (short A rows) and in the long-row kernels (A rows of
+    200 entries, ~50K products over 300,000 columns)."""
+    A, B = _mixed_run_operands(11)
+    C, info = run(A, B, mode=mode)
+    nnz = np.diff(C.row_ptr)
+    assert nnz.max() > 4096 and nnz.min() < 256  # long rows and hash rows both present
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
 def test_structure_is_deterministic():
     A = rt.gen_rmat(13, 16, seed=8)
     C1, _ = run(A, A)
