@@ -409,6 +409,10 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
           ok[r] = lane < __builtin_amdgcn_readlane(rem, j0 + r);
           src.load_at(lane_value(bs, j0 + r), ok[r] ? lane : 0, c[r], v[r]);
         }
+        // All ROWS gathers leave before any result is used.  Without this
+        // fence the scheduler moves the first product's use up between the
+        // loads, and the wait for it holds back the other seven.
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) p[r] = VAL ? lane_value(ae, j0 + r) * v[r] : 0.f;
         f(c, p, ok);
@@ -486,6 +490,7 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
         float v[ROWS], p[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) src.load(idx[r], c[r], v[r]);
+        __builtin_amdgcn_sched_barrier(0);  // as above: every gather in flight first
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) p[r] = VAL ? a[r] * v[r] : 0.f;
         f(c, p, ok);
@@ -2065,6 +2070,9 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
                          B.rows, heavy.nt, B.rp, B.col, heavy.slab.p);
       RT_HIP(hipGetLastError());
     }
+    // Rows that gather the same long B rows run together (their L2 reuse).
+    // Batches skip it: their B rows belong to different products.
+    if (!BATCH && heavy.n >= 1024) order_long_rows(A, B, heavy.list.p + heavy.off, heavy.n, s);
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     heavy.next.alloc(1);
     zero_slot(heavy.next.p, s);

@@ -70,6 +70,9 @@ void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipSt
                   RunStats* st);
 void bin_rows(const i64* x, i64 n, i64 cap, const i64 th[5], i32* list, int cnt[6], int off[7],
               hipStream_t s);
+// Reorders the long-row list `rows` (n entries) so that rows gathering the
+// same long B rows run together (k_order.hip).
+void order_long_rows(const Csr& A, const Csr& B, i32* rows, i32 n, hipStream_t s);
 void symbolic_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key, i64* nnz,
                    const Tuning& tu, hipStream_t s, HeavyRows& heavy);
 void numeric_pass(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
