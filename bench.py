@@ -111,7 +111,8 @@ def cpu_baseline(A, P, threads, target_s=15.0):
         oracle.stream_f32(S, A, 0, S.shape[0], threads)
         t = time.perf_counter() - t0
     return {"value": 2.0 * ps / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"1 row in {stride} of A ({S.shape[0]} rows, {ps:.3e} products), "
+            "sample": (f"all {S.shape[0]} rows of A (" if stride == 1 else
+                       f"1 row in {stride} of A ({S.shape[0]} rows, ") + f"{ps:.3e} products), "
                       f"FP32 Gustavson with sorted CSR rows, computed and discarded, {t:.1f} s"}
 
 
