@@ -18,6 +18,8 @@
 // counts and bitmaps, and the row's result does not depend on when it runs.
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "device.hpp"
 #include "spgemm.hpp"
 
@@ -26,12 +28,19 @@ using namespace dev;
 
 namespace {
 
-constexpr int HUBS = 32;
+// 16, 32 and 64 hubs measured the same on R-MAT scale 20 (long-row numeric
+// kernel 65.2-65.8 ms): the first few hubs carry the reuse.
+#ifndef RT_HUBS
+#define RT_HUBS 32
+#endif
+constexpr int HUBS = RT_HUBS;
+using HubKey = std::conditional_t<(HUBS > 32), u64, u32>;
+static_assert(HUBS <= 64, "hub keys are at most 64 bits");
 
 // Top HUBS of a list of keys, largest first: workgroup g sorts keys
 // [1024 g, 1024 g + 1024) in LDS (bitonic, descending) and writes its first
-// HUBS to out[HUBS g ...].  Rounds of this shrink any list 32-fold until one
-// workgroup holds it.  A key is (row length << 32) | ~row, so ties go to the
+// HUBS to out[HUBS g ...].  Rounds of this shrink any list 1024/HUBS-fold
+// until one workgroup holds it.  A key is (row length << 32) | ~row, so ties go to the
 // lower row and a zero key marks nothing.  With rp set, the keys are made
 // from B's row pointers (n = rows of B); otherwise read from in.
 constexpr int TOPK_WG = 1024;
@@ -83,23 +92,27 @@ __global__ __launch_bounds__(256) void k_hub_ranks(i64 nb, const u64* __restrict
   rank[i] = r;
 }
 
-// One workgroup per long row: bit (31 - h) of the row's key is set when
-// its A entries name hub h.
+// One workgroup per long row: bit (HUBS - 1 - h) of the row's key is set
+// when its A entries name hub h.
 __global__ __launch_bounds__(256) void k_hub_keys(const i32* __restrict__ rows,
                                                   const i64* __restrict__ arp,
                                                   const i32* __restrict__ acol,
                                                   const unsigned char* __restrict__ rank,
-                                                  u32* __restrict__ keys) {
-  __shared__ u32 s_or[256 / WAVE];
+                                                  HubKey* __restrict__ keys) {
+  __shared__ HubKey s_or[256 / WAVE];
   const i32 row = rows[blockIdx.x];
   const i64 e = arp[row + 1];
-  u32 key = 0;
+  HubKey key = 0;
   for (i64 j = arp[row] + threadIdx.x; j < e; j += 256) {
     const u32 h = rank[acol[j]];
-    if (h < HUBS) key |= 0x80000000u >> h;
+    if (h < HUBS) key |= (HubKey)1 << (HUBS - 1 - h);
   }
 #pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) key |= (u32)__shfl_xor((int)key, o, WAVE);
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    key |= (u32)__shfl_xor((int)(u32)key, o, WAVE);
+    if constexpr (sizeof(HubKey) == 8)
+      key |= (HubKey)(u32)__shfl_xor((int)(u32)(key >> 32), o, WAVE) << 32;
+  }
   if ((threadIdx.x & (WAVE - 1)) == 0) s_or[threadIdx.x / WAVE] = key;
   __syncthreads();
   if (threadIdx.x == 0) keys[blockIdx.x] = s_or[0] | s_or[1] | s_or[2] | s_or[3];
@@ -133,18 +146,18 @@ void order_long_rows(const Csr& A, const Csr& B, i32* rows, i32 n, hipStream_t s
                      dst, rank.p);
   RT_HIP(hipGetLastError());
   // Keys of the long rows, then the rows sorted by key, descending (stable).
-  DBuf<u32> keys(2 * (size_t)n);
+  DBuf<HubKey> keys(2 * (size_t)n);
   DBuf<i32> vals(2 * (size_t)n);
-  hipcub::DoubleBuffer<u32> kk(keys.p, keys.p + n);
+  hipcub::DoubleBuffer<HubKey> kk(keys.p, keys.p + n);
   hipcub::DoubleBuffer<i32> kv(vals.p, vals.p + n);
   size_t tb = 0;
-  RT_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, kk, kv, n, 0, 32, s));
+  RT_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, kk, kv, n, 0, HUBS, s));
   DBuf<unsigned char> tmp(tb);
   RT_HIP(hipMemcpyAsync(kv.Current(), rows, (size_t)n * sizeof(i32), hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_hub_keys, dim3((unsigned)n), dim3(256), 0, s, kv.Current(), A.rp, A.col,
                      rank.p, kk.Current());
   RT_HIP(hipGetLastError());
-  RT_HIP(hipcub::DeviceRadixSort::SortPairsDescending(tmp.p, tb, kk, kv, n, 0, 32, s));
+  RT_HIP(hipcub::DeviceRadixSort::SortPairsDescending(tmp.p, tb, kk, kv, n, 0, HUBS, s));
   RT_HIP(hipMemcpyAsync(rows, kv.Current(), (size_t)n * sizeof(i32), hipMemcpyDeviceToDevice, s));
   RT_HIP(hipStreamSynchronize(s));  // the scratch buffers are released at scope exit
 }
