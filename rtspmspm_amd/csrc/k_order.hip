@@ -40,9 +40,10 @@ static_assert(HUBS <= 64, "hub keys are at most 64 bits");
 // Top HUBS of a list of keys, largest first: workgroup g sorts keys
 // [1024 g, 1024 g + 1024) in LDS (bitonic, descending) and writes its first
 // HUBS to out[HUBS g ...].  Rounds of this shrink any list 1024/HUBS-fold
-// until one workgroup holds it.  A key is (row length << 32) | ~row, so ties go to the
-// lower row and a zero key marks nothing.  With rp set, the keys are made
-// from B's row pointers (n = rows of B); otherwise read from in.
+// until one workgroup holds it.  A key is (row length << 32) | ~row, so
+// ties go to the lower row and a zero key marks nothing.  With rp set, the
+// keys are made from B's row pointers (n = rows of B); otherwise they are
+// read from in.
 constexpr int TOPK_WG = 1024;
 
 __global__ __launch_bounds__(TOPK_WG) void k_topk(i64 n, const i64* __restrict__ rp,
