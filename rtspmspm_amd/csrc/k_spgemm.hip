@@ -2071,8 +2071,12 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
       RT_HIP(hipGetLastError());
     }
     // Rows that gather the same long B rows run together (their L2 reuse).
-    // Batches skip it: their B rows belong to different products.
-    if (!BATCH && heavy.n >= 1024) order_long_rows(A, B, heavy.list.p + heavy.off, heavy.n, s);
+    // Only where B (8 bytes an entry) outgrows the eight 4 MB L2s together:
+    // below that its rows stay cached in any order and the ordering's few
+    // launches cost more than they save.  Batches skip it: their B rows
+    // belong to different products.
+    if (!BATCH && heavy.n >= 1024 && B.nnz * 8 > ((i64)32 << 20))
+      order_long_rows(A, B, heavy.list.p + heavy.off, heavy.n, s);
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     heavy.next.alloc(1);
     zero_slot(heavy.next.p, s);

@@ -192,6 +192,37 @@ def test_chunk_walk_block_and_mark_modes(mode):
     check_parity(C, oracle.spgemm_f64(A, B))
 
 
+def test_long_row_order_with_hub_rows():
+    """B of 4.8M entries (38 MB, more than the L2s hold) with ten hub rows of
+    5,000 entries; 1,200 long rows, a third of them gathering hubs: the
+    long rows are reordered by the hubs they gather before both long-row
+    kernels run."""
+    rng = np.random.default_rng(21)
+    nb, ncols = 120_000, 1_000_000
+    lens = np.full(nb, 40)
+    lens[:10] = 5000
+    br = np.repeat(np.arange(nb), lens)
+    # Short rows: strictly increasing columns from gaps in [1, ncols/40).
+    gaps = rng.integers(1, ncols // 40, size=(nb - 10, 40))
+    bc = np.concatenate([rng.choice(ncols, n, replace=False) for n in lens[:10]] +
+                        [np.cumsum(gaps, axis=1).ravel()])
+    B = csr_from_coo(nb, ncols, br, bc, rng.uniform(-1, 1, br.size))
+    assert B.nnz * 8 > 32 << 20
+    na = 1200
+    ar, ac = [], []
+    for i in range(na):
+        ks = rng.choice(np.arange(10, nb), 110, replace=False)
+        if i % 3 == 0:
+            ks[:3] = rng.choice(10, 3, replace=False)
+        ar.append(np.full(ks.size, i))
+        ac.append(ks)
+    ar, ac = np.concatenate(ar), np.concatenate(ac)
+    A = csr_from_coo(na, nb, ar, ac, rng.uniform(-1, 1, ar.size))
+    C, _ = run(A, B)
+    assert np.diff(C.row_ptr).min() > 4096  # every row is a long row
+    check_parity(C, oracle.spgemm_f64(A, B))
+
+
 def test_structure_is_deterministic():
     A = rt.gen_rmat(13, 16, seed=8)
     C1, _ = run(A, A)
