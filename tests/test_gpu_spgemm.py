@@ -192,7 +192,8 @@ def test_chunk_walk_block_and_mark_modes(mode):
     check_parity(C, oracle.spgemm_f64(A, B))
 
 
-def test_long_row_order_with_hub_rows():
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_long_row_order_with_hub_rows(mode):
     """B of 4.8M entries (38 MB, more than the L2s hold) with ten hub rows of
     5,000 entries; 1,200 long rows, a third of them gathering hubs: the
     long rows are reordered by the hubs they gather before both long-row
@@ -218,7 +219,7 @@ def test_long_row_order_with_hub_rows():
         ac.append(ks)
     ar, ac = np.concatenate(ar), np.concatenate(ac)
     A = csr_from_coo(na, nb, ar, ac, rng.uniform(-1, 1, ar.size))
-    C, _ = run(A, B)
+    C, _ = run(A, B, mode=mode)
     assert np.diff(C.row_ptr).min() > 4096  # every row is a long row
     check_parity(C, oracle.spgemm_f64(A, B))
 
