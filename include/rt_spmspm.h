@@ -64,8 +64,12 @@ typedef struct rt_csr {
 /* Ablations of the faithful ray kernel.  They mirror the reference's compile
  * time variants (optixSpMSpM.cu:106-111 NOTHING, :151-189 NOINT,
  * :222-227 ARCHSUP, :228-229 NOMEM; src/CMakeLists.txt:348-376) as run-time
- * switches.  Only RT_ABL_FULL produces a product; the others produce a C with
- * the correct structure and unspecified values and exist for timing.        */
+ * switches and exist for timing.  Only RT_ABL_FULL produces a product.  With
+ * the dense output (RT_RAY_DENSE, and RT_RAY_AUTO where it picks dense) the
+ * other ablations touch no output entry, so C comes back with the right
+ * shape and no entries, as the reference's ablation builds leave their dense
+ * result all zero.  With RT_RAY_SLOT they return C's exact structure (from
+ * the symbolic pass) with unspecified values.                               */
 #define RT_ABL_FULL        0  /* ATOMIC: one global atomic add per product    */
 #define RT_ABL_NO_ATOMIC   1  /* ARCHSUP: non-atomic +1 into a 256-entry buffer */
 #define RT_ABL_TRAVERSE    2  /* NOMEM: walk B's row, accumulate nothing      */
@@ -77,15 +81,26 @@ typedef struct rt_csr {
 #define RT_RAY_DENSE       1  /* dense fp32 C + touch map, as the reference   */
 #define RT_RAY_SLOT        2  /* CSR slots located by the exact symbolic pass */
 
+/* Long rows (more than the last hash bin holds) in the learned method.     */
+#define RT_LONG_DEFAULT    0  /* the tuned choice (exact path untuned)        */
+#define RT_LONG_LEARNED    1  /* sized by the predictor, learned tile kernel   */
+#define RT_LONG_EXACT      2  /* symbolic bitmap + segmented numeric kernels   */
+
+/* rt_opts.flags.                                                           */
+#define RT_FLAG_TRUSTED    1  /* device operands are known-valid CSR: skip the
+                                 on-device structure check (k_validate).  A
+                                 malformed operand then reads out of bounds. */
+
 typedef struct rt_opts {
   int32_t mode;        /* RT_MODE_*                                         */
   int32_t ablation;    /* RT_ABL_* (RT_MODE_RAY only)                       */
   int32_t ray_output;  /* RT_RAY_* (RT_MODE_RAY only)                       */
   int32_t profile;     /* 1: time every kernel launch with hipEvents        */
   int32_t use_tuned;   /* 1: use/record autotuned bin thresholds            */
-  int32_t graph;       /* plain method: 0 = default (on for batched calls),
-                          1 = capture each stage in a hipGraph, 2 = direct */
-  int32_t reserved[2];
+  int32_t graph;       /* 0 = default (on for batched calls), 1 = capture
+                          each stage in a hipGraph, 2 = direct launches    */
+  int32_t learned_long;/* RT_MODE_LEARNED: RT_LONG_* handling of long rows */
+  int32_t flags;       /* RT_FLAG_* bits                                    */
   /* outputs */
   double   ms_total;       /* hipEvent time, inputs resident -> C complete  */
   int64_t  products;       /* intermediate products P                       */
@@ -130,8 +145,10 @@ int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_row
 typedef struct rt_plan rt_plan;
 int rt_plan_create(const rt_csr* A, const rt_csr* B, rt_plan** plan, rt_csr* C,
                    void* stream);
-/* a_val / b_val: device arrays of A.nnz / B.nnz new values; C must be the
- * rt_csr returned by rt_plan_create (its val array is overwritten).         */
+/* a_val / b_val: device arrays of at least A.nnz / B.nnz new values (the
+ * nnz of the operands given to rt_plan_create; shorter arrays are read past
+ * their end); C must be the rt_csr returned by rt_plan_create (its val array
+ * is overwritten).  ms (may be NULL) receives the device time.              */
 int rt_plan_execute(rt_plan* plan, const float* a_val, const float* b_val, rt_csr* C,
                     double* ms);
 void rt_plan_destroy(rt_plan* plan);
@@ -157,6 +174,19 @@ int rt_device_free(void* ptr);
 int rt_device_synchronize(void);
 /* Return cached device blocks of the library's pool to the driver.         */
 int rt_pool_trim(void);
+/* Bytes and blocks handed out by the pool and not yet returned, and bytes
+ * cached for reuse (any pointer may be NULL).                               */
+int rt_pool_stats(int64_t* live_bytes, int64_t* live_blocks, int64_t* cached_bytes);
+/* Test hook for the out-of-memory paths: the n-th pool allocation from now
+ * (n >= 1) fails as a failed hipMalloc does (RT_E_NOMEM), once; n = 0
+ * disarms it.  Lets tests check that every entry point then returns
+ * RT_E_NOMEM and releases what it had taken (rt_pool_stats).                */
+int rt_debug_fail_alloc(int64_t n);
+/* A HIP stream of the library's device at the default priority (returned as
+ * void*), for callers without HIP that run products concurrently from several
+ * host threads, one stream each.                                            */
+int rt_stream_create(void** stream);
+int rt_stream_destroy(void* stream);
 int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cus);
 
 /* ---- autotuner ------------------------------------------------------------
@@ -170,6 +200,15 @@ int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cu
  * No counterpart in the reference (its pipeline is fixed at build time).   */
 int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, char* desc,
                 int64_t desc_len);
+/* As rt_autotune for the batched call (rt_spmspm_batched with the same
+ * arguments): the batched class is tuned and cached separately.             */
+int rt_autotune_batched(int32_t count, const int64_t* a_rows, const int64_t* b_rows,
+                        const int64_t* b_cols, const rt_csr* A_stack, const rt_csr* B_stack,
+                        int32_t mode, int32_t reps, char* desc, int64_t desc_len);
+/* Reads a tuning-cache file as the library does and counts the lines it
+ * would use (*valid) and those it would drop as malformed, out of range or
+ * of another format version (*rejected).  Host only.                        */
+int rt_tune_cache_check(const char* path, int32_t* valid, int32_t* rejected);
 
 /* ---- per-kernel timing (rt_opts.profile = 1) -----------------------------
  * Kernel launch durations measured with hipEvents recorded on the stream
@@ -239,7 +278,12 @@ int rt_gen_batch_poisson(int32_t count, int64_t nmin, int64_t nmax, double lmin,
 int rt_mtx_read(const char* path, rt_csr* out);
 /* ref_compat = 0: every stored entry, row-major, 9 significant digits.
  * ref_compat = 1: the reference's printResult format (optixSpMSpM.cpp:452-494):
- *                 exact zeros dropped, 6 significant digits.                */
+ *                 exact zeros dropped, 6 significant digits, true count.
+ * ref_compat = 2: as 1, byte for byte what printResult leaves on disk: the
+ *                 size line is written as "rows cols " and later overwritten
+ *                 in place by "rows cols 0" (its entry map is never filled,
+ *                 :465,488-491), which is one byte longer and so replaces
+ *                 the first byte of the first entry line.                   */
 int rt_mtx_write(const char* path, const rt_csr* C, int32_t ref_compat);
 
 #ifdef __cplusplus

@@ -29,6 +29,8 @@ RT_OK, RT_E_INVALID, RT_E_SHAPE, RT_E_NOMEM, RT_E_HIP, RT_E_IO, RT_E_LIMIT, RT_E
     0, -1, -2, -3, -4, -5, -6, -7
 MEM_HOST, MEM_DEVICE = 0, 1
 MODES = {"plain": 0, "ray": 1, "learned": 2}
+LEARNED_LONG = {None: 0, "learned": 1, "exact": 2}
+FLAG_TRUSTED = 1
 ABLATIONS = {"full": 0, "no_atomic": 1, "traverse": 2, "no_hit": 3, "empty": 4}
 RAY_OUTPUTS = {"auto": 0, "dense": 1, "slot": 2}
 NFEAT = 8
@@ -50,7 +52,7 @@ class rt_opts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("ablation", ctypes.c_int32),
                 ("ray_output", ctypes.c_int32), ("profile", ctypes.c_int32),
                 ("use_tuned", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2),
+                ("learned_long", ctypes.c_int32), ("flags", ctypes.c_int32),
                 ("ms_total", ctypes.c_double), ("products", ctypes.c_int64),
                 ("overflow_rows", ctypes.c_int64), ("predicted_rows", ctypes.c_int64)]
 
@@ -80,9 +82,17 @@ _sig = {
     "rt_device_free": ([_c.c_void_p], _c.c_int),
     "rt_device_synchronize": ([], _c.c_int),
     "rt_pool_trim": ([], _c.c_int),
+    "rt_pool_stats": ([_P(_c.c_int64), _P(_c.c_int64), _P(_c.c_int64)], _c.c_int),
+    "rt_debug_fail_alloc": ([_c.c_int64], _c.c_int),
+    "rt_stream_create": ([_P(_c.c_void_p)], _c.c_int),
+    "rt_stream_destroy": ([_c.c_void_p], _c.c_int),
     "rt_device_info": ([_c.c_char_p, _c.c_int64, _P(_c.c_int64), _P(_c.c_int32)], _c.c_int),
     "rt_autotune": ([_P(rt_csr), _P(rt_csr), _c.c_int32, _c.c_int32, _c.c_char_p, _c.c_int64],
                     _c.c_int),
+    "rt_autotune_batched": ([_c.c_int32, _P(_c.c_int64), _P(_c.c_int64), _P(_c.c_int64),
+                             _P(rt_csr), _P(rt_csr), _c.c_int32, _c.c_int32, _c.c_char_p,
+                             _c.c_int64], _c.c_int),
+    "rt_tune_cache_check": ([_c.c_char_p, _P(_c.c_int32), _P(_c.c_int32)], _c.c_int),
     "rt_prof_reset": ([], None),
     "rt_prof_count": ([], _c.c_int32),
     "rt_prof_get": ([_c.c_int32, _c.c_char_p, _c.c_int64, _P(_c.c_int64), _P(_c.c_double)], _c.c_int),
@@ -192,6 +202,16 @@ class DeviceCSR:
         _check(_L.rt_memcpy_d2h(rp.ctypes.data, self._s.row_ptr, rp.nbytes))
         return rp
 
+    def segment(self, e0: int, e1: int):
+        """Entries [e0, e1) of col and val, copied to the host."""
+        n = int(e1) - int(e0)
+        col = np.zeros(max(n, 0), np.int32)
+        val = np.zeros(max(n, 0), np.float32)
+        if n > 0:
+            _check(_L.rt_memcpy_d2h(col.ctypes.data, self._s.col + 4 * int(e0), 4 * n))
+            _check(_L.rt_memcpy_d2h(val.ctypes.data, self._s.val + 4 * int(e0), 4 * n))
+        return col, val
+
     def to_host(self) -> CSR:
         h = rt_csr()
         _check(_L.rt_csr_to_host(ctypes.byref(self._s), ctypes.byref(h)))
@@ -218,7 +238,32 @@ def to_device(A: CSR) -> DeviceCSR:
 GRAPH = {None: 0, True: 1, False: 2}
 
 
-def _opts(mode, ablation, ray_output, profile, use_tuned, graph=None) -> rt_opts:
+class Stream:
+    """A HIP stream of the library's device (rt_stream_create), for running
+    products concurrently from several host threads."""
+
+    def __init__(self):
+        self.handle = _c.c_void_p()
+        _check(_L.rt_stream_create(ctypes.byref(self.handle)))
+
+    def close(self) -> None:
+        if self.handle:
+            _check(_L.rt_stream_destroy(self.handle))
+            self.handle = _c.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream(stream):
+    return None if stream is None else stream.handle
+
+
+def _opts(mode, ablation, ray_output, profile, use_tuned, graph=None, learned_long=None,
+          trusted=False) -> rt_opts:
     o = rt_opts()
     o.mode = MODES[mode]
     o.ablation = ABLATIONS[ablation]
@@ -226,6 +271,8 @@ def _opts(mode, ablation, ray_output, profile, use_tuned, graph=None) -> rt_opts
     o.profile = 1 if profile else 0
     o.use_tuned = 1 if use_tuned else 0
     o.graph = GRAPH[graph]
+    o.learned_long = LEARNED_LONG[learned_long]
+    o.flags = FLAG_TRUSTED if trusted else 0
     return o
 
 
@@ -235,38 +282,52 @@ def _info(o: rt_opts) -> dict:
 
 
 def spmspm(A, B, mode: str = "plain", ablation: str = "full", ray_output: str = "auto",
-           profile: bool = False, use_tuned: bool = False, graph=None):
+           profile: bool = False, use_tuned: bool = False, graph=None, learned_long=None,
+           trusted: bool = False, stream: Stream | None = None):
     """C = A*B.  Host CSR operands give a host CSR; DeviceCSR operands give a
     DeviceCSR.  Returns (C, info) with info = {ms, products, overflow_rows,
-    predicted_rows}; ms is the device time from inputs resident to C complete."""
-    o = _opts(mode, ablation, ray_output, profile, use_tuned, graph)
+    predicted_rows}; ms is the device time from inputs resident to C complete.
+    learned_long: None (tuned choice), "learned" or "exact" handling of the
+    learned method's long rows; trusted: skip the device check of device
+    operands; stream: run on this Stream instead of the library's."""
+    o = _opts(mode, ablation, ray_output, profile, use_tuned, graph, learned_long, trusted)
     C = rt_csr()
     if isinstance(A, DeviceCSR):
         if not isinstance(B, DeviceCSR):
             raise TypeError("A and B must both be DeviceCSR or both host CSR")
         _check(_L.rt_spmspm_ex(ctypes.byref(A._s), ctypes.byref(B._s), ctypes.byref(C),
-                               ctypes.byref(o), None))
+                               ctypes.byref(o), _stream(stream)))
         return DeviceCSR(C), _info(o)
     a, b = A._struct(), B._struct()
-    _check(_L.rt_spmspm_ex(ctypes.byref(a), ctypes.byref(b), ctypes.byref(C), ctypes.byref(o), None))
+    _check(_L.rt_spmspm_ex(ctypes.byref(a), ctypes.byref(b), ctypes.byref(C), ctypes.byref(o),
+                           _stream(stream)))
     return _take_host(C), _info(o)
+
+
+def _pairs(a_rows, b_rows, b_cols):
+    a_rows = np.ascontiguousarray(a_rows, np.int64)
+    b_rows = np.ascontiguousarray(b_rows, np.int64)
+    b_cols = np.ascontiguousarray(b_cols, np.int64)
+    if not (a_rows.size == b_rows.size == b_cols.size):
+        raise ValueError("a_rows, b_rows and b_cols must have one entry per pair")
+    return a_rows, b_rows, b_cols
 
 
 def spmspm_batched(a_rows, b_rows, b_cols, A_stack: DeviceCSR, B_stack: DeviceCSR,
                    mode: str = "plain", profile: bool = False, use_tuned: bool = False,
-                   graph=None):
+                   graph=None, learned_long=None, trusted: bool = False,
+                   stream: Stream | None = None):
     """Independent products of stacked device operands (see rt_spmspm_batched).
-    graph: None = library default (hipGraph stages for the plain method),
+    graph: None = library default (hipGraph stages for batched calls),
     True / False = force on / off."""
-    a_rows = np.ascontiguousarray(a_rows, np.int64)
-    b_rows = np.ascontiguousarray(b_rows, np.int64)
-    b_cols = np.ascontiguousarray(b_cols, np.int64)
-    o = _opts(mode, "full", "auto", profile, use_tuned, graph)
+    a_rows, b_rows, b_cols = _pairs(a_rows, b_rows, b_cols)
+    o = _opts(mode, "full", "auto", profile, use_tuned, graph, learned_long, trusted)
     C = rt_csr()
     _check(_L.rt_spmspm_batched(int(a_rows.size), a_rows.ctypes.data_as(_P(_c.c_int64)),
                                 b_rows.ctypes.data_as(_P(_c.c_int64)),
                                 b_cols.ctypes.data_as(_P(_c.c_int64)), ctypes.byref(A_stack._s),
-                                ctypes.byref(B_stack._s), ctypes.byref(C), ctypes.byref(o), None))
+                                ctypes.byref(B_stack._s), ctypes.byref(C), ctypes.byref(o),
+                                _stream(stream)))
     return DeviceCSR(C), _info(o)
 
 
@@ -285,6 +346,13 @@ class Plan:
     def execute(self, A_values: DeviceCSR, B_values: DeviceCSR) -> float:
         """New values are taken from the val arrays of two device CSRs with
         the plan's structure; returns the device time in ms."""
+        A, B = self._operands
+        for new, old, name in ((A_values, A, "A"), (B_values, B, "B")):
+            if not isinstance(new, DeviceCSR):
+                raise TypeError(f"{name}_values must be a DeviceCSR")
+            if new.shape != old.shape or new.nnz != old.nnz:
+                raise ValueError(f"{name}_values has shape {new.shape} and {new.nnz} entries; "
+                                 f"the plan's {name} has {old.shape} and {old.nnz}")
         ms = _c.c_double()
         _check(_L.rt_plan_execute(self._p, A_values._s.val, B_values._s.val,
                                   ctypes.byref(self.C._s), ctypes.byref(ms)))
@@ -294,6 +362,11 @@ class Plan:
         if self._p:
             _L.rt_plan_destroy(self._p)
             self._p = _c.c_void_p()
+
+    def free(self) -> None:
+        """Destroy the plan and release its C."""
+        self.close()
+        self.C.free()
 
     def __del__(self):
         try:
@@ -321,9 +394,39 @@ def autotune(A: DeviceCSR, B: DeviceCSR, mode: str = "plain", reps: int = 2) -> 
     """Time the candidate row-bin settings on these operands and cache the
     fastest for their matrix class (used with use_tuned=True).  Returns the
     tuner's one-line report."""
-    buf = ctypes.create_string_buffer(1024)
-    _check(_L.rt_autotune(ctypes.byref(A._s), ctypes.byref(B._s), MODES[mode], reps, buf, 1024))
+    buf = ctypes.create_string_buffer(2048)
+    _check(_L.rt_autotune(ctypes.byref(A._s), ctypes.byref(B._s), MODES[mode], reps, buf, 2048))
     return buf.value.decode()
+
+
+def autotune_batched(a_rows, b_rows, b_cols, A_stack: DeviceCSR, B_stack: DeviceCSR,
+                     mode: str = "plain", reps: int = 2) -> str:
+    """As autotune() for batched calls with these stacked operands."""
+    a_rows, b_rows, b_cols = _pairs(a_rows, b_rows, b_cols)
+    buf = ctypes.create_string_buffer(2048)
+    _check(_L.rt_autotune_batched(int(a_rows.size), a_rows.ctypes.data_as(_P(_c.c_int64)),
+                                  b_rows.ctypes.data_as(_P(_c.c_int64)),
+                                  b_cols.ctypes.data_as(_P(_c.c_int64)), ctypes.byref(A_stack._s),
+                                  ctypes.byref(B_stack._s), MODES[mode], reps, buf, 2048))
+    return buf.value.decode()
+
+
+def tune_cache_check(path: str):
+    """(valid, rejected) line counts of a tuning-cache file as the library reads it."""
+    v, r = _c.c_int32(), _c.c_int32()
+    _check(_L.rt_tune_cache_check(os.fsencode(path), ctypes.byref(v), ctypes.byref(r)))
+    return v.value, r.value
+
+
+def pool_stats() -> dict:
+    lb, lk, cb = _c.c_int64(), _c.c_int64(), _c.c_int64()
+    _check(_L.rt_pool_stats(ctypes.byref(lb), ctypes.byref(lk), ctypes.byref(cb)))
+    return {"live_bytes": lb.value, "live_blocks": lk.value, "cached_bytes": cb.value}
+
+
+def debug_fail_alloc(n: int) -> None:
+    """Make the n-th pool allocation from now fail (0 disarms); test hook."""
+    _check(_L.rt_debug_fail_alloc(int(n)))
 
 
 def synchronize() -> None:
@@ -416,9 +519,11 @@ def mtx_read(path: str) -> CSR:
     return _take_host(s)
 
 
-def mtx_write(path: str, C: CSR, ref_compat: bool = False) -> None:
+def mtx_write(path: str, C: CSR, ref_compat=False) -> None:
+    """ref_compat: False (every entry, 9 digits), True / 1 (the reference's
+    format with the true count) or 2 (also its in-place header rewrite)."""
     h = C._struct()
-    _check(_L.rt_mtx_write(os.fsencode(path), ctypes.byref(h), 1 if ref_compat else 0))
+    _check(_L.rt_mtx_write(os.fsencode(path), ctypes.byref(h), int(ref_compat)))
 
 
 # ---- learned predictor ----------------------------------------------------

@@ -2,6 +2,7 @@
 // Every entry point catches the library's exceptions and returns an RT_E*
 // code; rt_last_error() carries the message (the reference's equivalent is
 // the CUDA_CHECK/OPTIX_CHECK exception caught in main, optixSpMSpM.cpp:654-658).
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -45,35 +46,6 @@ int guard(F&& f) {
 
 void zero_csr(rt_csr* C) {
   if (C) std::memset(C, 0, sizeof(*C));
-}
-
-void check_csr_header(const rt_csr* M, const char* name) {
-  RT_CHECK(M != nullptr, RT_E_INVALID, std::string(name) + " is NULL");
-  RT_CHECK(M->rows >= 0 && M->cols >= 0 && M->nnz >= 0, RT_E_INVALID,
-           std::string(name) + " has negative dimensions");
-  RT_CHECK(M->mem == RT_MEM_HOST || M->mem == RT_MEM_DEVICE, RT_E_INVALID,
-           std::string(name) + " has an unknown memory space");
-  RT_CHECK(M->row_ptr != nullptr, RT_E_INVALID, std::string(name) + ".row_ptr is NULL");
-  RT_CHECK(M->nnz == 0 || (M->col != nullptr && M->val != nullptr), RT_E_INVALID,
-           std::string(name) + " has NULL col/val");
-  RT_CHECK(M->cols <= (i64)INT32_MAX, RT_E_LIMIT, std::string(name) + " has more than 2^31-1 columns");
-}
-
-// Full structural check of a host CSR: row_ptr monotone from 0 to nnz,
-// columns in range and strictly increasing inside each row.
-void validate_host(const rt_csr* M, const char* name) {
-  RT_CHECK(M->row_ptr[0] == 0 && M->row_ptr[M->rows] == M->nnz, RT_E_INVALID,
-           std::string(name) + ": row_ptr must run from 0 to nnz");
-  for (i64 i = 0; i < M->rows; ++i) {
-    const i64 a = M->row_ptr[i], b = M->row_ptr[i + 1];
-    RT_CHECK(a <= b, RT_E_INVALID, std::string(name) + ": row_ptr not monotone");
-    for (i64 e = a; e < b; ++e) {
-      RT_CHECK(M->col[e] >= 0 && M->col[e] < M->cols, RT_E_INVALID,
-               std::string(name) + ": column index out of range");
-      RT_CHECK(e == a || M->col[e] > M->col[e - 1], RT_E_INVALID,
-               std::string(name) + ": columns must be sorted and unique within a row");
-    }
-  }
 }
 
 Csr view(const rt_csr* M) {
@@ -160,24 +132,24 @@ struct EventPair {
   }
 };
 
-// One timed product with explicit settings (autotuner); returns device ms.
-double timed_product(const Csr& A, const Csr& B, int mode, const Tuning& tu, hipStream_t s) {
-  CsrOut C;
+// Runs one product with explicit settings (autotuner); returns the device
+// ms and leaves the result in C.
+double timed_product(const Csr& A, const Csr& B, const Batch& bt, int mode, const Tuning& tu,
+                     hipStream_t s, CsrOut& C) {
   RunStats st;
   EventPair ev;
   RT_HIP(hipEventRecord(ev.a, s));
   try {
-    if (mode == RT_MODE_LEARNED) spgemm_learned(A, B, Batch{}, C, tu, s, &st);
-    else spgemm_plain(A, B, Batch{}, C, tu, s, &st);
+    if (mode == RT_MODE_LEARNED) spgemm_learned(A, B, bt, C, tu, s, &st);
+    else if (bt.b_base) spgemm_plain_graph(A, B, bt, C, tu, s, &st);
+    else spgemm_plain(A, B, bt, C, tu, s, &st);
   } catch (...) {
     (void)hipStreamSynchronize(s);
     free_out(C);
     throw;
   }
   RT_HIP(hipEventRecord(ev.b, s));
-  const double ms = ev.ms();
-  free_out(C);
-  return ms;
+  return ev.ms();
 }
 
 // Runs the selected method on device operands; C receives pool arrays.
@@ -185,23 +157,27 @@ void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts&
                 hipStream_t s) {
   RunStats st;
   EventPair ev;
+  RT_CHECK(o.learned_long >= RT_LONG_DEFAULT && o.learned_long <= RT_LONG_EXACT, RT_E_INVALID,
+           "learned_long must be one of RT_LONG_*");
+  const bool batched = bt.b_base != nullptr;
+  Tuning tu = active_tuning(A, B, o.use_tuned != 0, o.mode, batched);
+  if (o.learned_long == RT_LONG_LEARNED) tu.learned_heavy = 1;
+  if (o.learned_long == RT_LONG_EXACT) tu.learned_heavy = 0;
   Prof::enabled = o.profile != 0;
   RT_HIP(hipEventRecord(ev.a, s));
   try {
-    const bool graph = o.graph == 1 || (o.graph == 0 && bt.b_base != nullptr);
+    const bool graph = o.graph == 1 || (o.graph == 0 && batched);
     switch (o.mode) {
       case RT_MODE_PLAIN:
-        if (graph)
-          spgemm_plain_graph(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0, o.mode), s, &st);
-        else
-          spgemm_plain(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0, o.mode), s, &st);
+        if (graph) spgemm_plain_graph(A, B, bt, C, tu, s, &st);
+        else spgemm_plain(A, B, bt, C, tu, s, &st);
         break;
       case RT_MODE_RAY:
-        RT_CHECK(bt.b_base == nullptr, RT_E_INVALID, "the ray method takes a single product");
+        RT_CHECK(!batched, RT_E_INVALID, "the ray method takes a single product");
         spgemm_ray(A, B, C, o.ablation, o.ray_output, s, &st);
         break;
       case RT_MODE_LEARNED:
-        spgemm_learned(A, B, bt, C, active_tuning(A, B, o.use_tuned != 0, o.mode), s, &st);
+        spgemm_learned(A, B, bt, C, tu, s, &st);
         break;
       default: fail(RT_E_INVALID, "unknown mode " + std::to_string(o.mode));
     }
@@ -218,6 +194,114 @@ void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts&
   o.products = st.products;
   o.overflow_rows = st.overflow_rows;
   o.predicted_rows = st.predicted_rows;
+}
+
+// Stacked batch operands: per-row B bases and column limits, validated
+// (unless trusted) and viewed as one product.
+struct BatchView {
+  DBuf<i64> base, alim, blim;
+  Csr A, B;
+  Batch bt;
+};
+
+void prepare_batch(int32_t count, const int64_t* a_rows, const int64_t* b_rows,
+                   const int64_t* b_cols, const rt_csr* A, const rt_csr* B, bool trusted,
+                   hipStream_t s, BatchView& v) {
+  RT_CHECK(a_rows && b_rows && b_cols, RT_E_INVALID, "NULL argument");
+  RT_CHECK(count >= 1, RT_E_INVALID, "count must be >= 1");
+  check_csr_header(A, "A_stack");
+  check_csr_header(B, "B_stack");
+  RT_CHECK(A->mem == RT_MEM_DEVICE && B->mem == RT_MEM_DEVICE, RT_E_INVALID,
+           "batched operands must be device memory");
+  i64 ar = 0, br = 0, maxk = 0, maxc = 0;
+  for (int b = 0; b < count; ++b) {
+    RT_CHECK(a_rows[b] >= 0 && b_rows[b] >= 0 && b_cols[b] >= 0, RT_E_INVALID, "negative pair size");
+    ar += a_rows[b];
+    br += b_rows[b];
+    maxk = std::max<i64>(maxk, b_rows[b]);
+    maxc = std::max<i64>(maxc, b_cols[b]);
+  }
+  RT_CHECK(ar == A->rows && br == B->rows, RT_E_SHAPE, "stacked row counts do not match the pair sizes");
+  RT_CHECK(A->cols >= maxk && B->cols >= maxc, RT_E_SHAPE, "stacked column counts too small");
+  device();
+  v.base.alloc(std::max<i64>(ar, 1));
+  {
+    std::vector<i64> bstart(count), av(count), bv(count);
+    i64 acc = 0;
+    for (int b = 0; b < count; ++b) {
+      bstart[b] = acc;
+      acc += b_rows[b];
+      av[b] = b_rows[b];
+      bv[b] = b_cols[b];
+    }
+    fill_segments(count, a_rows, bstart.data(), v.base.p, ar, s);
+    if (!trusted) {
+      v.alim.alloc(std::max<i64>(ar, 1));
+      v.blim.alloc(std::max<i64>(br, 1));
+      fill_segments(count, a_rows, av.data(), v.alim.p, ar, s);
+      fill_segments(count, b_rows, bv.data(), v.blim.p, br, s);
+    }
+  }
+  v.A = view(A);
+  v.B = view(B);
+  v.A.cols = maxk;
+  if (!trusted) {
+    validate_csr_dev(v.A, A->cols, v.alim.p, s, "A_stack");
+    validate_csr_dev(v.B, B->cols, v.blim.p, s, "B_stack");
+  }
+  v.B.cols = maxc;
+  v.bt.b_base = v.base.p;
+}
+
+// The tuner proper: times the candidates of autotune_search on (A, B, bt),
+// rejects any whose C differs from the first candidate's (structure hashes
+// exactly; value sums beyond 1e-3 of sum |v|, far above the rounding of a
+// different accumulation order), records the fastest under `key`.
+std::string tune_operands(const Csr& a, const Csr& b, const Batch& bt, int mode, int reps,
+                          const std::string& key, hipStream_t s) {
+  bool have_ref = false;
+  Fingerprint ref;
+  auto eval = [&](const Tuning& t) -> double {
+    {
+      CsrOut C;  // warm-up (allocations, caches), and the result check
+      timed_product(a, b, bt, mode, t, s, C);
+      Fingerprint f;
+      try {
+        f = fingerprint(C, s);
+      } catch (...) {
+        free_out(C);
+        throw;
+      }
+      free_out(C);
+      if (!have_ref) {
+        ref = f;
+        have_ref = true;
+      } else {
+        const double tol = 1e-3 * ref.vsum[0] + 1e-30;
+        if (f.nnz != ref.nnz || f.hrp != ref.hrp || f.hcol != ref.hcol ||
+            std::abs(f.vsum[0] - ref.vsum[0]) > tol || std::abs(f.vsum[1] - ref.vsum[1]) > tol)
+          return -1.0;
+      }
+    }
+    double t_ms = 0;
+    for (int r = 0; r < reps; ++r) {
+      CsrOut C;
+      t_ms += timed_product(a, b, bt, mode, t, s, C);
+      free_out(C);
+    }
+    return t_ms / reps;
+  };
+  std::string log;
+  const Tuning best = autotune_search(mode == RT_MODE_LEARNED, eval, log);
+  record_tuning(key, best);
+  return "class " + key + " " + log + "; " + tuning_report(best);
+}
+
+void copy_desc(const std::string& line, char* desc, int64_t desc_len) {
+  if (desc && desc_len > 0) {
+    std::strncpy(desc, line.c_str(), desc_len - 1);
+    desc[desc_len - 1] = '\0';
+  }
 }
 
 }  // namespace
@@ -252,8 +336,10 @@ int rt_spmspm_ex(const rt_csr* A, const rt_csr* B, rt_csr* C, rt_opts* opts, voi
       run_device(dA.v, dB.v, Batch{}, Cd, o, s);
       export_host(Cd, C, s);
     } else {
-      validate_csr_dev(view(A), A->cols, nullptr, s, "A");
-      validate_csr_dev(view(B), B->cols, nullptr, s, "B");
+      if (!(o.flags & RT_FLAG_TRUSTED)) {
+        validate_csr_dev(view(A), A->cols, nullptr, s, "A");
+        validate_csr_dev(view(B), B->cols, nullptr, s, "B");
+      }
       run_device(view(A), view(B), Batch{}, Cd, o, s);
       export_device(Cd, C);
     }
@@ -265,52 +351,20 @@ int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_row
                       rt_opts* opts, void* stream) {
   zero_csr(C);
   return guard([&] {
-    RT_CHECK(C != nullptr && a_rows && b_rows && b_cols, RT_E_INVALID, "NULL argument");
-    RT_CHECK(count >= 1, RT_E_INVALID, "count must be >= 1");
-    check_csr_header(A, "A_stack");
-    check_csr_header(B, "B_stack");
-    RT_CHECK(A->mem == RT_MEM_DEVICE && B->mem == RT_MEM_DEVICE, RT_E_INVALID,
-             "batched operands must be device memory");
-    i64 ar = 0, br = 0, maxk = 0, maxc = 0;
-    for (int b = 0; b < count; ++b) {
-      RT_CHECK(a_rows[b] >= 0 && b_rows[b] >= 0 && b_cols[b] >= 0, RT_E_INVALID, "negative pair size");
-      ar += a_rows[b];
-      br += b_rows[b];
-      maxk = std::max<i64>(maxk, b_rows[b]);
-      maxc = std::max<i64>(maxc, b_cols[b]);
-    }
-    RT_CHECK(ar == A->rows && br == B->rows, RT_E_SHAPE, "stacked row counts do not match the pair sizes");
-    RT_CHECK(A->cols >= maxk && B->cols >= maxc, RT_E_SHAPE, "stacked column counts too small");
+    RT_CHECK(C != nullptr, RT_E_INVALID, "NULL argument");
     rt_opts local{};
     rt_opts& o = opts ? *opts : local;
     RT_CHECK(o.mode == RT_MODE_PLAIN || o.mode == RT_MODE_LEARNED, RT_E_INVALID,
              "batched products support the plain and learned methods");
-    device();
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : lib_stream();
-    // Per-row B base and per-row column limits of both operands.
-    DBuf<i64> base(std::max<i64>(ar, 1)), alim(std::max<i64>(ar, 1)), blim(std::max<i64>(br, 1));
-    {
-      std::vector<i64> bstart(count), av(count), bv(count);
-      i64 acc = 0;
-      for (int b = 0; b < count; ++b) {
-        bstart[b] = acc;
-        acc += b_rows[b];
-        av[b] = b_rows[b];
-        bv[b] = b_cols[b];
-      }
-      fill_segments(count, a_rows, bstart.data(), base.p, ar, s);
-      fill_segments(count, a_rows, av.data(), alim.p, ar, s);
-      fill_segments(count, b_rows, bv.data(), blim.p, br, s);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : nullptr;
+    if (!s) {
+      device();
+      s = lib_stream();
     }
-    Csr Av = view(A), Bv = view(B);
-    Av.cols = maxk;
-    validate_csr_dev(Av, A->cols, alim.p, s, "A_stack");
-    validate_csr_dev(Bv, B->cols, blim.p, s, "B_stack");
-    Bv.cols = maxc;
+    BatchView v;
+    prepare_batch(count, a_rows, b_rows, b_cols, A, B, (o.flags & RT_FLAG_TRUSTED) != 0, s, v);
     CsrOut Cd;
-    Batch bt;
-    bt.b_base = base.p;
-    run_device(Av, Bv, bt, Cd, o, s);
+    run_device(v.A, v.B, v.bt, Cd, o, s);
     export_device(Cd, C);
   });
 }
@@ -331,35 +385,34 @@ int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, ch
     const Csr a = view(A), b = view(B);
     validate_csr_dev(a, A->cols, nullptr, s, "A");
     validate_csr_dev(b, B->cols, nullptr, s, "B");
-    const std::vector<Tuning> cand = tune_candidates(mode == RT_MODE_LEARNED);
-    size_t best = 0;
-    double best_ms = 0;
-    std::string report;
-    for (size_t i = 0; i < cand.size(); ++i) {
-      timed_product(a, b, mode, cand[i], s);  // warm-up (allocations, caches)
-      double t = 0;
-      for (int r = 0; r < reps; ++r) t += timed_product(a, b, mode, cand[i], s);
-      t /= reps;
-      report += (i ? ", " : "") + std::to_string(i) + ":" + std::to_string(t);
-      if (i == 0 || t < best_ms) {
-        best = i;
-        best_ms = t;
-      }
-    }
-    const std::string key = tune_key(a, b, mode);
-    record_tuning(key, cand[best]);
-    std::string chosen = "; thresholds sym";
-    for (i64 v : cand[best].sym) chosen += " " + std::to_string(v);
-    chosen += " num";
-    for (i64 v : cand[best].num) chosen += " " + std::to_string(v);
-    chosen += " learned_heavy " + std::to_string(cand[best].learned_heavy);
-    const std::string line = "class " + key + " best candidate " + std::to_string(best) + " (" +
-                             std::to_string(best_ms) + " ms); ms per candidate: " + report +
-                             chosen;
-    if (desc && desc_len > 0) {
-      std::strncpy(desc, line.c_str(), desc_len - 1);
-      desc[desc_len - 1] = '\0';
-    }
+    copy_desc(tune_operands(a, b, Batch{}, mode, reps, tune_key(a, b, mode, false), s), desc,
+              desc_len);
+  });
+}
+
+int rt_autotune_batched(int32_t count, const int64_t* a_rows, const int64_t* b_rows,
+                        const int64_t* b_cols, const rt_csr* A, const rt_csr* B, int32_t mode,
+                        int32_t reps, char* desc, int64_t desc_len) {
+  return guard([&] {
+    RT_CHECK(mode == RT_MODE_PLAIN || mode == RT_MODE_LEARNED, RT_E_INVALID,
+             "tune the plain or the learned method");
+    RT_CHECK(reps >= 1, RT_E_INVALID, "reps must be >= 1");
+    device();
+    hipStream_t s = lib_stream();
+    BatchView v;
+    prepare_batch(count, a_rows, b_rows, b_cols, A, B, false, s, v);
+    copy_desc(tune_operands(v.A, v.B, v.bt, mode, reps, tune_key(v.A, v.B, mode, true), s), desc,
+              desc_len);
+  });
+}
+
+int rt_tune_cache_check(const char* path, int32_t* valid, int32_t* rejected) {
+  return guard([&] {
+    RT_CHECK(path != nullptr, RT_E_INVALID, "NULL path");
+    int v = 0, r = 0;
+    tune_cache_check(path, v, r);
+    if (valid) *valid = v;
+    if (rejected) *rejected = r;
   });
 }
 
@@ -471,6 +524,40 @@ int rt_device_synchronize(void) {
 
 int rt_pool_trim(void) {
   return guard([&] { pool_trim(); });
+}
+
+int rt_pool_stats(int64_t* live_bytes, int64_t* live_blocks, int64_t* cached_bytes) {
+  return guard([&] {
+    i64 lb = 0, lk = 0, cb = 0;
+    pool_stats(lb, lk, cb);
+    if (live_bytes) *live_bytes = lb;
+    if (live_blocks) *live_blocks = lk;
+    if (cached_bytes) *cached_bytes = cb;
+  });
+}
+
+int rt_debug_fail_alloc(int64_t n) {
+  return guard([&] {
+    RT_CHECK(n >= 0, RT_E_INVALID, "n must be >= 0");
+    pool_fail_after(n);
+  });
+}
+
+int rt_stream_create(void** stream) {
+  return guard([&] {
+    RT_CHECK(stream != nullptr, RT_E_INVALID, "NULL argument");
+    *stream = nullptr;
+    device();
+    hipStream_t st = nullptr;
+    RT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *stream = st;
+  });
+}
+
+int rt_stream_destroy(void* stream) {
+  return guard([&] {
+    if (stream) RT_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  });
 }
 
 int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cus) {

@@ -4,7 +4,7 @@
 //
 //   optixSpMSpM [-m1 A.mtx] [-m2 B.mtx] [-o C.mtx] [-l log] [-h] [--dim=WxH]
 //               [--mode ray|plain|learned] [--ablation full|no_atomic|traverse|no_hit|empty]
-//               [--ref-compat]
+//               [--ref-compat] [--ref-header]
 //
 // Same flags, defaults (Matrix/test1.mtx, Matrix/test2.mtx, Matrix/result.mtx,
 // optixSpMSpM.cpp:502-504) and log contract: the -l file receives one number,
@@ -34,6 +34,7 @@ void usage(const char* argv0) {
             << "         --mode ray|plain|learned    SpGEMM method (default ray)\n"
             << "         --ablation full|no_atomic|traverse|no_hit|empty   (ray method)\n"
             << "         --ref-compat                write C as the reference does\n"
+            << "         --ref-header                ... including its in-place header rewrite\n"
             << "         --help | -h                 Print this usage message\n";
   std::exit(1);
 }
@@ -70,6 +71,7 @@ int main(int argc, char* argv[]) {
     else if (a == "--log" || a == "-l") log = next();
     else if (a.rfind("--dim=", 0) == 0) { /* image size of the reference; unused */ }
     else if (a == "--ref-compat") ref_compat = 1;
+    else if (a == "--ref-header") ref_compat = 2;
     else if (a == "--mode") {
       const std::string v = next();
       if (v == "ray") mode = RT_MODE_RAY;

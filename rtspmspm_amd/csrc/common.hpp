@@ -42,6 +42,12 @@ void set_last_error(const std::string& msg);
     if (!(cond)) ::rt::fail((code), (msg));  \
   } while (0)
 
+// Host checks of an rt_csr argument (capi.cpp): the header fields, and the
+// full structure of a host CSR (row_ptr from 0 to nnz, monotone; columns in
+// range, sorted and unique within each row).  Both throw RT_E_INVALID.
+void check_csr_header(const rt_csr* M, const char* name);
+void validate_host(const rt_csr* M, const char* name);
+
 // ---- device properties ---------------------------------------------------
 struct DeviceInfo {
   int id = 0;
@@ -61,6 +67,9 @@ void* dmalloc(size_t bytes);
 void dfree(void* p);
 void pool_trim();
 size_t pool_cached_bytes();
+void pool_stats(i64& live_bytes, i64& live_blocks, i64& cached_bytes);
+// Test hook: the n-th dmalloc from now fails with RT_E_NOMEM (0 disarms).
+void pool_fail_after(i64 n);
 
 // RAII device buffer from the pool.
 template <class T>
@@ -95,9 +104,11 @@ struct DBuf {
 hipStream_t lib_stream();
 
 // When profiling is on, launch() brackets the kernel with hipEvents recorded
-// on `s` and accumulates its duration under `name`.
+// on `s` and accumulates its duration under `name`.  The switch and the open
+// bracket are per host thread, so that concurrent calls on different streams
+// time only their own launches; the accumulated table is shared.
 struct Prof {
-  static bool enabled;
+  static thread_local bool enabled;
   static void begin(const char* name, hipStream_t s);
   static void end(hipStream_t s);
   static void flush();  // resolve pending event pairs (synchronises)

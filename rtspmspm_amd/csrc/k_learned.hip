@@ -261,7 +261,7 @@ struct BinTop {
 __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__ prod, i64 ncols,
                                                   const float* __restrict__ pred, float margin,
                                                   BinTop bt, i64 th, int learned_heavy,
-                                                  i64* __restrict__ cap,
+                                                  int bins_mask, i64* __restrict__ cap,
                                                   i64* __restrict__ key_light,
                                                   i64* __restrict__ key_heavy,
                                                   i64* __restrict__ key_exact,
@@ -271,6 +271,7 @@ __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__
   const i64 P = prod[i];
   const i64 U = min(P, ncols);
   i64 c = 0;
+  int bin = 5;
   if (P > 0) {
     const float e = fminf(pred[i] + margin, 40.f);
     c = (i64)ceilf(exp2f(e) - 1.f);
@@ -279,11 +280,14 @@ __global__ __launch_bounds__(256) void k_capacity(i64 m, const i64* __restrict__
     for (int b = 0; b < 5; ++b)
       if (c <= bt.t[b]) {
         c = min(bt.t[b], U);
+        bin = b;
         break;
       }
   }
   i64 ke = 0;
-  if (U > th && !learned_heavy) {  // long row on the exact path
+  // Long rows on the exact path, and rows predicted into a bin the tuning
+  // leaves to the exact path.
+  if (P > 0 && ((U > th && !learned_heavy) || (U <= th && (bin > 4 || !((bins_mask >> bin) & 1))))) {
     ke = P;
     c = 0;
   }
@@ -464,6 +468,11 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                     const Tuning& tu, hipStream_t s, RunStats* st) {
   const i64 m = A.rows;
   RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
+  if (tu.learned_bins == 0 && !tu.learned_heavy) {
+    // The tuning predicts no row: every row takes the exact path.
+    spgemm_plain(A, B, batch, C, tu, s, st);
+    return;
+  }
   std::vector<float> hp;
   float margin = 0;
   current_weights(hp, margin);
@@ -490,7 +499,8 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
     BinTop bt;
     for (int b = 0; b < 5; ++b) bt.t[b] = tu.num[b];
     hipLaunchKernelGGL(k_capacity, g256, dim3(256), 0, s, m, prod.p, B.cols, pred.p, margin, bt,
-                       std::min(tu.sym[4], tu.num[4]), tu.learned_heavy, cap.p, key_light.p,
+                       std::min(tu.sym[4], tu.num[4]), tu.learned_heavy, tu.learned_bins, cap.p,
+                       key_light.p,
                        key_heavy.p,
                        key_exact.p, ovf.p);
     RT_HIP(hipGetLastError());
@@ -601,8 +611,12 @@ int rt_mlp_get_weights(float* params, int64_t n, float* margin) {
 int rt_row_features(const rt_csr* A, const rt_csr* B, float* feat, int64_t* row_nnz) {
   return guard([&] {
     RT_CHECK(A && B && feat, RT_E_INVALID, "NULL argument");
+    check_csr_header(A, "A");
+    check_csr_header(B, "B");
     RT_CHECK(A->mem == RT_MEM_HOST && B->mem == RT_MEM_HOST, RT_E_INVALID, "host operands expected");
     RT_CHECK(A->cols == B->rows, RT_E_SHAPE, "shape mismatch");
+    validate_host(A, "A");  // before any upload: a malformed CSR never reaches a kernel
+    validate_host(B, "B");
     device();
     hipStream_t s = lib_stream();
     const i64 m = A->rows, M = std::max<i64>(m, 1);

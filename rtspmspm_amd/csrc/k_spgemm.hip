@@ -1976,6 +1976,74 @@ void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStre
            "columns must be in range, sorted and unique within each row)");
 }
 
+// Fingerprint of a CSR result, to compare two runs of one product: wrapping
+// sums of a 64-bit mix of (index, row pointer) and of (position, column)
+// -- equal exactly when the structures are equal, up to hash collisions --
+// and two double sums of the values, sum |v| and sum v * w(position) with a
+// pseudo-random weight in [-1/2, 1/2), which differ between runs only by
+// rounding of the FP32 accumulation order.
+__device__ __forceinline__ u64 mix64(u64 x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_fingerprint(i64 rows, i64 nnz, const i64* __restrict__ rp,
+                                                     const i32* __restrict__ col,
+                                                     const float* __restrict__ val,
+                                                     unsigned long long* __restrict__ hsum,
+                                                     double* __restrict__ dsum) {
+  __shared__ u64 s_u[4];
+  __shared__ double s_d[4];
+  u64 hr = 0, hc = 0;
+  double va = 0, vd = 0;
+  const i64 stride = (i64)gridDim.x * 256;
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i <= rows; i += stride)
+    hr += mix64(((u64)i << 40) ^ (u64)rp[i]);
+  for (i64 e = (i64)blockIdx.x * 256 + threadIdx.x; e < nnz; e += stride) {
+    const u64 m = mix64((u64)e);
+    hc += mix64(m ^ (u64)(u32)col[e]);
+    const double v = val[e];
+    va += v < 0 ? -v : v;
+    vd += v * ((double)(m >> 11) * 0x1p-53 - 0.5);
+  }
+  hr = block_reduce_sum<256>(hr, s_u);
+  hc = block_reduce_sum<256>(hc, s_u);
+  va = block_reduce_sum<256>(va, s_d);
+  vd = block_reduce_sum<256>(vd, s_d);
+  if (threadIdx.x == 0) {
+    atomicAdd(&hsum[0], (unsigned long long)hr);
+    atomicAdd(&hsum[1], (unsigned long long)hc);
+    atomicAdd(&dsum[0], va);
+    atomicAdd(&dsum[1], vd);
+  }
+}
+
+Fingerprint fingerprint(const CsrOut& C, hipStream_t s) {
+  DBuf<unsigned long long> h(2);
+  DBuf<double> d(2);
+  RT_HIP(hipMemsetAsync(h.p, 0, 2 * sizeof(unsigned long long), s));
+  RT_HIP(hipMemsetAsync(d.p, 0, 2 * sizeof(double), s));
+  const i64 n = std::max<i64>(C.rows + 1, C.nnz);
+  const unsigned grid = (unsigned)std::max<i64>(1, std::min<i64>((n + 255) / 256,
+                                                                  (i64)device().cus * 8));
+  hipLaunchKernelGGL(k_fingerprint, dim3(grid), dim3(256), 0, s, C.rows, C.nnz, C.rp, C.col, C.val,
+                     h.p, d.p);
+  RT_HIP(hipGetLastError());
+  Fingerprint f;
+  unsigned long long hh[2];
+  RT_HIP(hipMemcpyAsync(hh, h.p, sizeof(hh), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipMemcpyAsync(f.vsum, d.p, sizeof(f.vsum), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipStreamSynchronize(s));
+  f.hrp = hh[0];
+  f.hcol = hh[1];
+  f.nnz = C.nnz;
+  return f;
+}
+
 // ---------------------------------------------------------------------------
 // Host pipeline pieces (shared by the plain and learned methods).
 namespace {
@@ -1992,19 +2060,47 @@ void launch_hash(const char* name, HashArgs a, const i32* list, i32 n, hipStream
   RT_HIP(hipGetLastError());
 }
 
+// The five hash bins, each in the workgroup shape the tuning picks:
+// lanes per row G, table slots T, workgroup WG (G == WG: one row per
+// workgroup).  Default shapes, then the alternates (see hash_group).
+template <int MODE, bool BATCH, class L>
+void for_hash_bins(const int shape[5], L&& launch) {
+  if (shape[0]) launch.template go<16, 32, 256, MODE, BATCH>(0);
+  else launch.template go<8, 32, 256, MODE, BATCH>(0);
+  if (shape[1]) launch.template go<32, 128, 256, MODE, BATCH>(1);
+  else launch.template go<16, 128, 256, MODE, BATCH>(1);
+  if (shape[2]) launch.template go<32, 512, 256, MODE, BATCH>(2);
+  else launch.template go<64, 512, 256, MODE, BATCH>(2);
+  if (shape[3]) launch.template go<256, 2048, 256, MODE, BATCH>(3);
+  else launch.template go<128, 2048, 128, MODE, BATCH>(3);
+  if (shape[4]) launch.template go<1024, 8192, 1024, MODE, BATCH>(4);
+  else launch.template go<512, 8192, 512, MODE, BATCH>(4);
+}
+
+// Host-listed bins: bin b's rows are list[off[b] .. off[b] + cnt[b]).
+struct ListedBins {
+  const HashArgs& a;
+  const i32* list;
+  const int* cnt;
+  const int* off;
+  const char* const* names;
+  hipStream_t s;
+  template <int G, int T, int WG, int M, bool BT>
+  void go(int b) {
+    launch_hash<G, T, WG, M, BT>(names[b], a, list + off[b], cnt[b], s);
+  }
+};
+
 template <int MODE, bool BATCH>
 void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const int* off,
-                      hipStream_t s) {
+                      const int shape[5], hipStream_t s) {
   static const char* names[3][5] = {
       {"sym_hash_t32", "sym_hash_t128", "sym_hash_t512", "sym_hash_t2048", "sym_hash_t8192"},
       {"num_hash_t32", "num_hash_t128", "num_hash_t512", "num_hash_t2048", "num_hash_t8192"},
       {"learn_hash_t32", "learn_hash_t128", "learn_hash_t512", "learn_hash_t2048",
        "learn_hash_t8192"}};
-  launch_hash<8, 32, 256, MODE, BATCH>(names[MODE][0], a, list + off[0], cnt[0], s);
-  launch_hash<16, 128, 256, MODE, BATCH>(names[MODE][1], a, list + off[1], cnt[1], s);
-  launch_hash<64, 512, 256, MODE, BATCH>(names[MODE][2], a, list + off[2], cnt[2], s);
-  launch_hash<128, 2048, 128, MODE, BATCH>(names[MODE][3], a, list + off[3], cnt[3], s);
-  launch_hash<512, 8192, 512, MODE, BATCH>(names[MODE][4], a, list + off[4], cnt[4], s);
+  ListedBins l{a, list, cnt, off, names[MODE], s};
+  for_hash_bins<MODE, BATCH>(shape, l);
 }
 
 // int32 B offsets (and eight products per lane in flight) when B's entries
@@ -2044,7 +2140,7 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
   bin_rows(key, m, B.cols, tu.sym, heavy.list.p, cnt, off, s);
   HashArgs a = make_args(A, B, bbase);
   a.nnz = nnz;
-  launch_hash_bins<MODE_SYM, BATCH>(a, heavy.list.p, cnt, off, s);
+  launch_hash_bins<MODE_SYM, BATCH>(a, heavy.list.p, cnt, off, tu.shape, s);
   heavy.off = off[5];
   heavy.n = cnt[5];
   heavy.nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
@@ -2136,14 +2232,14 @@ void numeric_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     a.crp = C.rp;
     a.ccol = C.col;
     a.cval = C.val;
-    launch_hash_bins<MODE_NUM, BATCH>(a, list.p, cnt, off, s);
+    launch_hash_bins<MODE_NUM, BATCH>(a, list.p, cnt, off, tu.shape, s);
     RT_HIP(hipStreamSynchronize(s));  // slab is released at scope exit
   } else {
     HashArgs a = make_args(A, B, bbase);
     a.crp = C.rp;
     a.ccol = C.col;
     a.cval = C.val;
-    launch_hash_bins<MODE_NUM, BATCH>(a, list.p, cnt, off, s);
+    launch_hash_bins<MODE_NUM, BATCH>(a, list.p, cnt, off, tu.shape, s);
     RT_HIP(hipStreamSynchronize(s));
   }
 }
@@ -2164,7 +2260,7 @@ void learned_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key, 
   a.cap = cap;
   a.ovf = ovf;
   a.nnz = nnz;
-  launch_hash_bins<MODE_LEARN, BATCH>(a, list.p, cnt, off, s);
+  launch_hash_bins<MODE_LEARN, BATCH>(a, list.p, cnt, off, tu.shape, s);
   RT_HIP(hipStreamSynchronize(s));
 }
 
@@ -2380,13 +2476,23 @@ void launch_hash_dev(HashArgs a, const DevBins& b, int bin, i64 max_rows, hipStr
   RT_HIP(hipGetLastError());
 }
 
+// Device-binned rows (persistent kernels, counts read on the device).
+struct DeviceBins {
+  const HashArgs& a;
+  const DevBins& b;
+  i64 max_rows;
+  hipStream_t s;
+  template <int G, int T, int WG, int M, bool BT>
+  void go(int bin) {
+    launch_hash_dev<G, T, WG, M, BT>(a, b, bin, max_rows, s);
+  }
+};
+
 template <int MODE, bool BATCH>
-void launch_hash_bins_dev(const HashArgs& a, const DevBins& b, i64 max_rows, hipStream_t s) {
-  launch_hash_dev<8, 32, 256, MODE, BATCH>(a, b, 0, max_rows, s);
-  launch_hash_dev<16, 128, 256, MODE, BATCH>(a, b, 1, max_rows, s);
-  launch_hash_dev<64, 512, 256, MODE, BATCH>(a, b, 2, max_rows, s);
-  launch_hash_dev<128, 2048, 128, MODE, BATCH>(a, b, 3, max_rows, s);
-  launch_hash_dev<512, 8192, 512, MODE, BATCH>(a, b, 4, max_rows, s);
+void launch_hash_bins_dev(const HashArgs& a, const DevBins& b, i64 max_rows, const int shape[5],
+                          hipStream_t s) {
+  DeviceBins l{a, b, max_rows, s};
+  for_hash_bins<MODE, BATCH>(shape, l);
 }
 
 // Captures the launches `body` issues on `s` into a graph, runs it once and
@@ -2457,7 +2563,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       bin_rows_dev(prod.p, m, B.cols, tu.sym, sb, s);
       HashArgs a = base;
       a.nnz = nnz.p;
-      launch_hash_bins_dev<MODE_SYM, BATCH>(a, sb, m, s);
+      launch_hash_bins_dev<MODE_SYM, BATCH>(a, sb, m, tu.shape, s);
       if (B.rows > 0)
         hipLaunchKernelGGL(k_slab_ptrs,
                            dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)), dim3(256), 0,
@@ -2487,7 +2593,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       a.crp = C.rp;
       a.ccol = C.col;
       a.cval = C.val;
-      launch_hash_bins_dev<MODE_NUM, BATCH>(a, nb, m, s);
+      launch_hash_bins_dev<MODE_NUM, BATCH>(a, nb, m, tu.shape, s);
       RT_HIP(hipGetLastError());
     });
   } catch (...) {

@@ -13,7 +13,8 @@
 // row-major, exact zeros dropped (:479), default ostream precision of 6
 // significant digits (:484).  The reference re-writes its header in place
 // with nnz = 0 (:465,488-491), which overwrites the start of the first entry
-// line; here the header carries the true count.  ref_compat = 0 writes every
+// line; ref_compat = 1 writes the true count instead, and ref_compat = 2
+// reproduces the overwrite byte for byte.  ref_compat = 0 writes every
 // stored entry, including cancellation zeros, with 9 significant digits, so
 // that fp32 values round-trip exactly.
 #include <algorithm>
@@ -77,9 +78,13 @@ void mtx_read(const char* path, HostCsr& out) {
   std::vector<i64> ri;
   std::vector<i32> ci;
   std::vector<float> vi;
-  ri.reserve(nnz * (symmetric ? 2 : 1));
-  ci.reserve(nnz * (symmetric ? 2 : 1));
-  vi.reserve(nnz * (symmetric ? 2 : 1));
+  // Room for the declared entries, but never more than the file can hold
+  // (an entry line has at least 4 bytes): a bogus count cannot make the
+  // reader reserve memory the entries will not use.
+  const long long room = std::min<long long>(nnz, (long long)(end - p) / 4 + 1) * (symmetric ? 2 : 1);
+  ri.reserve(room);
+  ci.reserve(room);
+  vi.reserve(room);
   while (p < end) {
     if (*p == '%' || *p == '\n' || *p == '\r') {
       p = next_line(p);
@@ -150,18 +155,33 @@ void mtx_read(const char* path, HostCsr& out) {
 }
 
 void mtx_write(const char* path, const rt_csr& C, int ref_compat) {
+  RT_CHECK(ref_compat >= 0 && ref_compat <= 2, RT_E_INVALID, "ref_compat must be 0, 1 or 2");
   FILE* f = std::fopen(path, "wb");
   if (!f) fail(RT_E_IO, std::string("cannot open ") + path + " for writing");
   i64 count = 0;
   for (i64 e = 0; e < C.nnz; ++e) count += ref_compat ? (C.val[e] != 0.0f) : 1;
-  std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n");
-  std::fprintf(f, "%lld %lld %lld\n", (long long)C.rows, (long long)C.cols, (long long)count);
+  const char* head = "%%MatrixMarket matrix coordinate real general\n";
+  std::fputs(head, f);
+  if (ref_compat == 2)  // printResult's first size line: no count (optixSpMSpM.cpp:470)
+    std::fprintf(f, "%lld %lld \n", (long long)C.rows, (long long)C.cols);
+  else
+    std::fprintf(f, "%lld %lld %lld\n", (long long)C.rows, (long long)C.cols, (long long)count);
   const char* fmt = ref_compat ? "%lld %lld %g\n" : "%lld %lld %.9g\n";
   for (i64 i = 0; i < C.rows; ++i)
     for (i64 e = C.row_ptr[i]; e < C.row_ptr[i + 1]; ++e) {
       if (ref_compat && C.val[e] == 0.0f) continue;
       std::fprintf(f, fmt, (long long)(i + 1), (long long)C.col[e] + 1, (double)C.val[e]);
     }
+  if (ref_compat == 2) {
+    // The in-place rewrite with the never-filled map's size, 0
+    // (optixSpMSpM.cpp:487-491).
+    if (std::fseek(f, 0, SEEK_SET) != 0) {
+      std::fclose(f);
+      fail(RT_E_IO, std::string("seek failed: ") + path);
+    }
+    std::fputs(head, f);
+    std::fprintf(f, "%lld %lld 0\n", (long long)C.rows, (long long)C.cols);
+  }
   if (std::fclose(f) != 0) fail(RT_E_IO, std::string("write failed: ") + path);
 }
 

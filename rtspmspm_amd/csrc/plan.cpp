@@ -65,6 +65,8 @@ int rt_plan_create(const rt_csr* A, const rt_csr* B, rt_plan** plan, rt_csr* C, 
   if (C) *C = rt_csr{};
   return guard([&] {
     RT_CHECK(A && B && plan && C, RT_E_INVALID, "NULL argument");
+    check_csr_header(A, "A");
+    check_csr_header(B, "B");
     RT_CHECK(A->mem == RT_MEM_DEVICE && B->mem == RT_MEM_DEVICE, RT_E_INVALID,
              "plans take device operands");
     RT_CHECK(A->cols == B->rows, RT_E_SHAPE, "shape mismatch");
@@ -83,8 +85,8 @@ int rt_plan_create(const rt_csr* A, const rt_csr* B, rt_plan** plan, rt_csr* C, 
     RT_HIP(hipMemsetAsync(p->nnz.p, 0, std::max<i64>(m, 1) * sizeof(i64), p->stream));
     symbolic_pass(p->A, p->B, nullptr, prod.p, p->nnz.p, p->tune, p->stream, p->heavy);
     CsrOut Cd;
-    alloc_c(Cd, p->nnz.p, m, B->cols, p->stream);
     try {
+      alloc_c(Cd, p->nnz.p, m, B->cols, p->stream);
       numeric_pass(p->A, p->B, nullptr, p->nnz.p, p->heavy, Cd, p->tune, p->stream);
     } catch (...) {
       (void)hipStreamSynchronize(p->stream);

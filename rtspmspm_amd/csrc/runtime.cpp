@@ -1,4 +1,4 @@
-// Error state, device properties, pool allocator, library stream and the
+// Device properties, pool allocator, library stream and the
 // per-kernel hipEvent timer behind rt_prof_*.
 #include <algorithm>
 #include <cstring>
@@ -8,13 +8,6 @@
 #include "common.hpp"
 
 namespace rt {
-
-static thread_local std::string g_last_error;
-
-void set_last_error(const std::string& msg) { g_last_error = msg; }
-const char* last_error_cstr() { return g_last_error.c_str(); }
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw Error(code, msg); }
 
 const DeviceInfo& device() {
   static DeviceInfo info;
@@ -47,6 +40,8 @@ struct Pool {
   std::multimap<size_t, void*> free_blocks;  // size -> ptr
   std::map<void*, size_t> live;              // ptr -> size
   size_t cached = 0;
+  size_t live_bytes = 0;
+  i64 fail_after = 0;  // test hook: > 0 counts down to one injected failure
 };
 Pool& pool() {
   static Pool* p = new Pool();  // never destroyed: blocks outlive static dtors
@@ -64,6 +59,9 @@ void* dmalloc(size_t bytes) {
   Pool& P = pool();
   {
     std::lock_guard<std::mutex> g(P.mu);
+    if (P.fail_after > 0 && --P.fail_after == 0)
+      fail(RT_E_NOMEM, "device allocation of " + std::to_string(sz) +
+                           " bytes failed (injected by rt_debug_fail_alloc)");
     auto it = P.free_blocks.lower_bound(sz);
     if (it != P.free_blocks.end() && it->first <= sz + sz / 4) {
       void* ptr = it->second;
@@ -71,6 +69,7 @@ void* dmalloc(size_t bytes) {
       P.free_blocks.erase(it);
       P.cached -= got;
       P.live[ptr] = got;
+      P.live_bytes += got;
       return ptr;
     }
   }
@@ -87,6 +86,7 @@ void* dmalloc(size_t bytes) {
   }
   std::lock_guard<std::mutex> g(P.mu);
   P.live[ptr] = sz;
+  P.live_bytes += sz;
   return ptr;
 }
 
@@ -98,6 +98,7 @@ void dfree(void* ptr) {
   if (it == P.live.end()) return;  // not ours
   P.free_blocks.emplace(it->second, ptr);
   P.cached += it->second;
+  P.live_bytes -= it->second;
   P.live.erase(it);
 }
 
@@ -115,6 +116,20 @@ size_t pool_cached_bytes() {
   return pool().cached;
 }
 
+void pool_stats(i64& live_bytes, i64& live_blocks, i64& cached_bytes) {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  live_bytes = (i64)P.live_bytes;
+  live_blocks = (i64)P.live.size();
+  cached_bytes = (i64)P.cached;
+}
+
+void pool_fail_after(i64 n) {
+  Pool& P = pool();
+  std::lock_guard<std::mutex> g(P.mu);
+  P.fail_after = n > 0 ? n : 0;
+}
+
 // ---- stream ------------------------------------------------------------------
 hipStream_t lib_stream() {
   static hipStream_t s = nullptr;
@@ -127,7 +142,7 @@ hipStream_t lib_stream() {
 }
 
 // ---- profiling ---------------------------------------------------------------
-bool Prof::enabled = false;
+thread_local bool Prof::enabled = false;
 namespace {
 struct ProfEntry {
   i64 launches = 0;
@@ -141,8 +156,8 @@ std::mutex g_prof_mu;
 std::vector<std::pair<std::string, ProfEntry>> g_prof;
 std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_event_cache;
-std::string g_cur_name;
-hipEvent_t g_cur_start;
+thread_local std::string g_cur_name;  // the open bracket of this thread
+thread_local hipEvent_t g_cur_start;
 
 hipEvent_t get_event() {
   if (!g_event_cache.empty()) {

@@ -1,5 +1,7 @@
 // Internal interface between the C ABI (capi.cpp) and the device pipelines.
 #pragma once
+#include <string>
+
 #include "common.hpp"
 
 namespace rt {
@@ -27,19 +29,35 @@ struct Batch {
   const i64* b_base = nullptr;  // per A row
 };
 
-// Row-bin thresholds of the two-phase method (inclusive upper bounds).
-// Symbolic bins use U = min(products, cols); numeric bins use exact nnz.
-// Bins 0..4 are LDS-hash kernels with tables of 32/128/512/2048/8192 slots
-// (symbolic: keys; numeric: keys + values); bin 5 (above sym[4] / num[4]) is
-// the LDS-bitmap symbolic kernel / the dense LDS column-tile numeric kernel.
+// Row-bin settings of the two-phase method.  Thresholds are inclusive upper
+// bounds: symbolic bins use U = min(products, cols); numeric bins use exact
+// nnz.  Bins 0..4 are LDS-hash kernels with tables of 32/128/512/2048/8192
+// slots (symbolic: keys; numeric: keys + values), so a threshold sets the
+// bin's load factor (at most 1/2: a numeric row's sorted entries are staged
+// in half the table); bin 5 (above sym[4] / num[4]) is the LDS-bitmap
+// symbolic kernel / the long-row numeric kernels.
 // Invariant: sym[4] <= num[4] so every numeric bin-5 row has tile counts.
+constexpr int HASH_T[5] = {32, 128, 512, 2048, 8192};
 struct Tuning {
   i64 sym[5] = {16, 64, 256, 1024, 4096};
   i64 num[5] = {16, 64, 256, 1024, 4096};
   // Learned method: long rows (U > num[4]) sized by the predictor and run by
   // the learned dense-tile kernel (1), or sent to the exact path (0).
   int learned_heavy = 0;
+  // Per hash bin, the workgroup shape: 0 the default lanes per row (8, 16,
+  // 64, 128, 512 for bins 0..4), 1 the alternate (16, 32, 32, 256, 1024).
+  int shape[5] = {0, 0, 0, 0, 0};
+  // Learned method: numeric bins (bit b) whose rows are sized by the
+  // predictor; rows predicted into other bins run the exact path.
+  int learned_bins = 31;
 };
+// Lanes per row of hash bin b in shape `alt`.
+inline int hash_group(int b, int alt) {
+  static const int g[2][5] = {{8, 16, 64, 128, 512}, {16, 32, 32, 256, 1024}};
+  return g[alt ? 1 : 0][b];
+}
+// Empty string when t is usable by the kernels, else what is wrong with it.
+std::string tuning_problem(const Tuning& t);
 
 struct RunStats {
   i64 products = 0;
@@ -108,6 +126,14 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
 // Per-row features (n x RT_NFEAT, device) and optionally exact nnz.
 void row_features(const Csr& A, const Csr& B, const Batch& batch, float* feat,
                   const i64* prod, hipStream_t s);
+
+// Fingerprint of a result (k_fingerprint): structure hashes and value sums.
+struct Fingerprint {
+  i64 nnz = 0;
+  u64 hrp = 0, hcol = 0;
+  double vsum[2] = {0, 0};  // sum |v|, sum v * w(position)
+};
+Fingerprint fingerprint(const CsrOut& C, hipStream_t s);
 
 // Device utilities.
 void exclusive_scan_i64(const i64* in, i64* out /* n+1 */, i64 n, hipStream_t s);

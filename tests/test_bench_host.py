@@ -11,8 +11,10 @@ import bench  # noqa: E402
 
 
 def test_parse_thresholds_takes_the_numeric_cut():
-    line = ("class r20c20a4b4SP best candidate 2 (114.5 ms); ms per candidate: 0:120.4, "
-            "1:114.7; thresholds sym 16 64 256 512 1024 num 16 64 256 512 1024 learned_heavy 0")
+    line = ("class r20c20a4b4SP best 114.5 ms of 10 candidates; ms per candidate: default:120.4, "
+            "cut2048:114.7; bins (slots/max nnz/lanes): 32/16/8 128/64/16 512/256/64 2048/512/128 "
+            "8192/1024/512; thresholds sym 16 64 256 512 1024 num 16 64 256 512 1024 "
+            "learned_heavy 0 shapes 0 0 0 0 0 learned_bins 31")
     assert bench.parse_thresholds(line) == [16, 64, 256, 512, 1024]
 
 

@@ -88,16 +88,38 @@ def test_cli_ref_compat_and_errors(tmp_path):
 
 
 def test_autotune_records_and_is_used(tmp_path, monkeypatch):
-    monkeypatch.setenv("RTSPMSPM_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    """One class each for single and batched calls: the report names the
+    chosen table sizes, thresholds and lanes per row, no candidate's result
+    differed from the first's, and the cached choice runs at parity."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("RTSPMSPM_TUNE_CACHE", str(cache))
     A = rt.gen_rmat(13, 16, seed=61)
     dA = rt.to_device(A)
+    ref = oracle.spgemm_f64(A, A)
+    keys = []
     for mode in ("plain", "learned"):
         report = rt.autotune(dA, dA, mode=mode, reps=1)
-        assert report.startswith("class ") and "best candidate" in report
+        print(report)
+        key = report.split()[1]
+        assert report.startswith("class ") and key.endswith("S" + mode[0].upper())
+        assert "bins (slots/max nnz/lanes): 32/" in report and "rejected" not in report
+        keys.append(key)
         C, _ = rt.spmspm(A, A, mode=mode, use_tuned=True)
-        check_parity(C, oracle.spgemm_f64(A, A))
-    lines = (tmp_path / "tune.txt").read_text().split("\n")
-    assert len([x for x in lines if x.strip()]) == 2  # one class, one line per method
+        check_parity(C, ref)
+    ar, br, bc, SA, SB = rt.gen_batch_poisson(24, 64, 4000, 4.0, 32.0, 62)
+    dSA, dSB = rt.to_device(SA), rt.to_device(SB)
+    report = rt.autotune_batched(ar, br, bc, dSA, dSB, mode="learned", reps=1)
+    print(report)
+    key = report.split()[1]
+    assert key.endswith("BL") and "rejected" not in report
+    keys.append(key)
+    Ct = rt.spmspm_batched(ar, br, bc, dSA, dSB, mode="learned", use_tuned=True)[0].to_host()
+    Cp = rt.spmspm_batched(ar, br, bc, dSA, dSB, mode="plain")[0].to_host()
+    assert np.array_equal(Ct.row_ptr, Cp.row_ptr) and np.array_equal(Ct.col, Cp.col)
+    lines = [x for x in cache.read_text().split("\n") if x.strip()]
+    assert sorted(x.split()[1] for x in lines) == sorted(keys)
+    assert all(x.startswith("v2 ") for x in lines)
+    assert rt.tune_cache_check(str(cache)) == (3, 0)
 
 
 def test_ae_driver_writes_harness_tables(tmp_path):

@@ -146,3 +146,98 @@ def test_cli_usage_exits_1():
     assert r.returncode == 1 and "--mat1" in r.stderr
     r = subprocess.run([exe, "--bogus"], capture_output=True, text=True)
     assert r.returncode == 1 and "Unknown option" in r.stderr
+
+
+def test_mtx_ref_header_reproduces_printresult_overwrite(tmp_path):
+    # printResult (optixSpMSpM.cpp:452-494) writes the size line as
+    # "rows cols " and, after the entries, rewrites the header in place with
+    # the size of a map it never filled: "rows cols 0".  That line is one byte
+    # longer, so it eats the first byte of the first entry line.
+    head = "%%MatrixMarket matrix coordinate real general\n"
+    C = rt.CSR((2, 3), np.array([0, 2, 3]), np.array([0, 2, 1], np.int32),
+               np.array([0.0, 1.0 / 3.0, 2.5], np.float32))
+    p = tmp_path / "c.mtx"
+    rt.mtx_write(str(p), C, ref_compat=2)
+    first = head + "2 3 \n" + "1 3 0.333333\n" + "2 2 2.5\n"
+    over = head + "2 3 0\n"
+    want = over + first[len(over):]
+    assert p.read_bytes().decode() == want
+    assert want == head + "2 3 0\n 3 0.333333\n2 2 2.5\n"
+    # no entries: the rewrite extends the file by its extra byte
+    E = rt.CSR((4, 5), np.zeros(5, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    rt.mtx_write(str(p), E, ref_compat=2)
+    assert p.read_bytes().decode() == head + "4 5 0\n"
+    with pytest.raises(rt.RtError) as e:
+        rt.mtx_write(str(p), C, ref_compat=3)
+    assert e.value.code == rt.RT_E_INVALID
+
+
+def test_tune_cache_lines_are_validated(tmp_path):
+    good = "v2 r20c20a4b4SP 16 64 256 1024 2048 16 64 256 1024 2048 0 0 1 0 0 1 31"
+    bad = [
+        "r20c20a4b4SP 16 64 256 1024 2048 16 64 256 1024 2048 0",        # old format, no version
+        "v1 r20c20a4b4SP 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0 31",
+        "v2 k 17 64 256 1024 2048 17 64 256 1024 2048 0 0 0 0 0 0 31",     # 17 > 32/2
+        "v2 k 16 64 256 1024 8192 16 64 256 1024 8192 0 0 0 0 0 0 31",     # 8192 > 8192/2
+        "v2 k 16 64 64 1024 2048 16 64 64 1024 2048 0 0 0 0 0 0 31",       # not increasing
+        "v2 k 16 64 256 1024 4096 16 64 256 1024 2048 0 0 0 0 0 0 31",     # sym[4] > num[4]
+        "v2 k 0 64 256 1024 2048 0 64 256 1024 2048 0 0 0 0 0 0 31",       # not positive
+        "v2 k 16 64 256 1024 2048 16 64 256 1024 2048 2 0 0 0 0 0 31",     # learned_heavy 2
+        "v2 k 16 64 256 1024 2048 16 64 256 1024 2048 0 0 3 0 0 0 31",     # shape 3
+        "v2 k 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0 40",     # bins mask
+        "v2 k 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0",        # field missing
+        "v2 k 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0 31 7",   # extra field
+        "v2 k sixteen 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0 31",
+    ]
+    p = tmp_path / "tune.txt"
+    p.write_text("# comment\n\n" + "\n".join([good] + bad) + "\n")
+    assert rt.tune_cache_check(str(p)) == (1, len(bad))
+    assert rt.tune_cache_check(str(tmp_path / "absent.txt")) == (0, 0)
+
+
+@pytest.mark.parametrize("case", ["col_range", "unsorted", "rowptr_start", "rowptr_order"])
+def test_row_features_rejects_malformed_operands_on_host(case):
+    # The checks run before any upload or launch, so no GPU is needed.
+    rp = np.array([0, 2, 3], np.int64)
+    col = np.array([0, 2, 1], np.int32)
+    if case == "col_range":
+        col = np.array([0, 3, 1], np.int32)
+    elif case == "unsorted":
+        col = np.array([2, 0, 1], np.int32)
+    elif case == "rowptr_start":
+        rp = np.array([1, 2, 3], np.int64)
+    else:
+        rp = np.array([0, 3, 2], np.int64)
+    bad = rt.CSR((2, 3), rp, col, np.ones(3, np.float32))
+    good = rt.CSR((3, 3), np.array([0, 1, 2, 3]), np.array([0, 1, 2], np.int32),
+                  np.ones(3, np.float32))
+    with pytest.raises(rt.RtError) as e:
+        rt.row_features(bad, good)
+    assert e.value.code == rt.RT_E_INVALID
+
+
+def test_plan_create_rejects_null_row_pointer():
+    import ctypes
+    lib = ctypes.CDLL(rt.LIB_PATH)
+    A = rt.rt_csr(4, 4, 0, None, None, None, rt.MEM_DEVICE, 0)
+    plan = ctypes.c_void_p()
+    C = rt.rt_csr()
+    rc = lib.rt_plan_create(ctypes.byref(A), ctypes.byref(A), ctypes.byref(plan),
+                            ctypes.byref(C), None)
+    assert rc == rt.RT_E_INVALID and not plan.value
+    assert b"row_ptr is NULL" in rt._L.rt_last_error()
+
+
+@pytest.mark.parametrize("target,exe", [("rtspmspm_amd/csrc", "asan_host"),
+                                        ("oracle", "oracle_asan")])
+def test_host_code_under_address_sanitizer(tmp_path, target, exe):
+    """The CPU-side code (generators, Matrix Market I/O, argument checks,
+    tuning-cache parser; and the oracle) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer and driven through its cases."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, target), "asan"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([os.path.join(REPO, "_build", "asan", exe), str(tmp_path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
+    assert "checks: ok" in r.stdout
