@@ -133,6 +133,31 @@ int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_row
                       const rt_csr* B_stack, rt_csr* C_stack,
                       rt_opts* opts, void* stream);
 
+/* ---- product without device allocation ----------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the exact two-phase product of device operands A and B
+ * written into the caller's arrays, with every scratch buffer in one
+ * file-scope device array of RT_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends.  No hipMalloc, hipFree or allocating library call is made.
+ *   c_row_ptr  device array of A.rows + 1 entries (always written when the
+ *              product fits the scratch)
+ *   c_col, c_val  device arrays of c_capacity entries (may be NULL when
+ *              c_capacity is 0)
+ *   *c_nnz     nnz(C); -1 when the product needs more scratch than there is
+ * Returns RT_E_LIMIT when C needs more than c_capacity entries (c_row_ptr
+ * then holds C's row pointer and c_col / c_val are untouched: call again
+ * with arrays of *c_nnz entries), and RT_E_LIMIT with a message when the
+ * operands need more scratch than RT_NOALLOC_SCRATCH_BYTES (rows of A and B
+ * up to a few hundred thousand fit).  flags: RT_FLAG_TRUSTED skips the
+ * device check of the operands.  Calls on one GPU share its scratch: within
+ * a process they are serialised; calls that overlap on one GPU from several
+ * processes race.  Synchronous on `stream` (NULL: the library's stream).    */
+#define RT_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int rt_spmspm_noalloc(const rt_csr* A, const rt_csr* B, int64_t* c_row_ptr, int32_t* c_col,
+                      float* c_val, int64_t c_capacity, int64_t* c_nnz, int32_t flags,
+                      void* stream);
+
 /* ---- numeric-only re-run on a fixed structure ---------------------------
  * Counterpart of cuSPARSE's SpGEMMreuse path used by the reference's
  * comparator when built with REUSE (cuSparse/src/main.cpp:98-363, switch at

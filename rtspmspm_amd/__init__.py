@@ -69,6 +69,8 @@ _sig = {
     "rt_spmspm_ex": ([_P(rt_csr), _P(rt_csr), _P(rt_csr), _P(rt_opts), _c.c_void_p], _c.c_int),
     "rt_spmspm_batched": ([_c.c_int32, _P(_c.c_int64), _P(_c.c_int64), _P(_c.c_int64),
                            _P(rt_csr), _P(rt_csr), _P(rt_csr), _P(rt_opts), _c.c_void_p], _c.c_int),
+    "rt_spmspm_noalloc": ([_P(rt_csr), _P(rt_csr), _c.c_void_p, _c.c_void_p, _c.c_void_p,
+                           _c.c_int64, _P(_c.c_int64), _c.c_int32, _c.c_void_p], _c.c_int),
     "rt_plan_create": ([_P(rt_csr), _P(rt_csr), _P(_c.c_void_p), _P(rt_csr), _c.c_void_p], _c.c_int),
     "rt_plan_execute": ([_c.c_void_p, _c.c_void_p, _c.c_void_p, _P(rt_csr), _P(_c.c_double)], _c.c_int),
     "rt_plan_destroy": ([_c.c_void_p], None),

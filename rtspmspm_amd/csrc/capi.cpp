@@ -17,8 +17,6 @@ const char* last_error_cstr();
 void prof_reset();
 int prof_count();
 bool prof_get(int i, std::string& name, i64& launches, double& ms);
-void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
-                      const char* what);
 double device_copy(void* dst, const void* src, i64 bytes, hipStream_t s);
 }  // namespace rt
 
@@ -366,6 +364,36 @@ int rt_spmspm_batched(int32_t count, const int64_t* a_rows, const int64_t* b_row
     CsrOut Cd;
     run_device(v.A, v.B, v.bt, Cd, o, s);
     export_device(Cd, C);
+  });
+}
+
+int rt_spmspm_noalloc(const rt_csr* A, const rt_csr* B, int64_t* c_row_ptr, int32_t* c_col,
+                      float* c_val, int64_t c_capacity, int64_t* c_nnz, int32_t flags,
+                      void* stream) {
+  if (c_nnz) *c_nnz = -1;
+  return guard([&] {
+    check_csr_header(A, "A");
+    check_csr_header(B, "B");
+    RT_CHECK(A->mem == RT_MEM_DEVICE && B->mem == RT_MEM_DEVICE, RT_E_INVALID,
+             "rt_spmspm_noalloc takes device operands");
+    RT_CHECK(A->cols == B->rows, RT_E_SHAPE,
+             "shape mismatch: A is " + std::to_string(A->rows) + "x" + std::to_string(A->cols) +
+                 ", B is " + std::to_string(B->rows) + "x" + std::to_string(B->cols));
+    RT_CHECK(c_row_ptr != nullptr && c_capacity >= 0, RT_E_INVALID,
+             "c_row_ptr is NULL or c_capacity negative");
+    RT_CHECK(c_capacity == 0 || (c_col && c_val), RT_E_INVALID, "NULL c_col / c_val");
+    device();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    i64 n = -1;
+    try {
+      spgemm_noalloc(view(A), view(B), c_row_ptr, c_col, c_val, c_capacity, &n,
+                     (flags & RT_FLAG_TRUSTED) != 0, s);
+    } catch (...) {
+      if (c_nnz) *c_nnz = n;
+      (void)hipStreamSynchronize(s);
+      throw;
+    }
+    if (c_nnz) *c_nnz = n;
   });
 }
 

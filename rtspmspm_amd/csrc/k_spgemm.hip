@@ -11,6 +11,10 @@
 // accumulate becomes an insert into an LDS table owned by output row i.
 #include <algorithm>
 #include <climits>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "device.hpp"
 #include "spgemm.hpp"
@@ -1883,20 +1887,26 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const i64* __restrict__ in,
   }
 }
 
-void exclusive_scan_i64(const i64* in, i64* out, i64 n, hipStream_t s) {
-  const i64 nb = std::max<i64>(1, (n + SCAN_TILE - 1) / SCAN_TILE);
-  DBuf<i64> bsum(nb);
+i64 scan_partials(i64 n) { return std::max<i64>(1, (n + SCAN_TILE - 1) / SCAN_TILE); }
+
+void exclusive_scan_i64(const i64* in, i64* out, i64 n, hipStream_t s, i64* partials) {
+  const i64 nb = scan_partials(n);
+  DBuf<i64> own;
+  if (!partials) {
+    own.alloc(nb);
+    partials = own.p;
+  }
   {
     ProfScope ps("scan_reduce", s);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, s, in, n, bsum.p);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, s, in, n, partials);
   }
   {
     ProfScope ps("scan_top", s);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nb, out + n);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, partials, nb, out + n);
   }
   if (n > 0) {
     ProfScope ps("scan_apply", s);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, s, in, n, bsum.p, out);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, s, in, n, partials, out);
   }
   RT_HIP(hipGetLastError());
   // bsum is released to the pool here; the pool only hands it out again to
@@ -1959,18 +1969,22 @@ __global__ __launch_bounds__(256) void k_validate(i64 rows, i64 nnz, const i64* 
 }
 
 void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
-                      const char* what) {
-  DBuf<int> flag(1);
-  RT_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+                      const char* what, int* flag) {
+  DBuf<int> own;
+  if (!flag) {
+    own.alloc(1);
+    flag = own.p;
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, flag, (i64)1, 0);
   if (M.rows > 0) {
     ProfScope ps("validate", s);
     hipLaunchKernelGGL(k_validate, dim3((unsigned)((M.rows * 16 + 255) / 256)), dim3(256), 0, s,
-                       M.rows, M.nnz, M.rp, M.col, col_limit, row_limit, flag.p);
+                       M.rows, M.nnz, M.rp, M.col, col_limit, row_limit, flag);
     RT_HIP(hipGetLastError());
   } else {
     RT_CHECK(M.rows == 0, RT_E_INVALID, "negative rows");
   }
-  const int f = read_scalar(flag.p, s);
+  const int f = read_scalar(flag, s);
   RT_CHECK(f == 0, RT_E_INVALID,
            std::string(what) + ": malformed CSR (row_ptr must run monotonically from 0 to nnz; "
            "columns must be in range, sorted and unique within each row)");
@@ -2267,25 +2281,29 @@ void learned_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key, 
 }  // namespace
 
 void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int cnt[6],
-              int off[7], hipStream_t s) {
+              int off[7], hipStream_t s, i32* counters) {
   Thr th;
   for (int b = 0; b < 5; ++b) th.t[b] = th_in[b];
-  DBuf<i32> dcnt(NBIN);
-  RT_HIP(hipMemsetAsync(dcnt.p, 0, NBIN * sizeof(i32), s));
+  DBuf<i32> own;
+  if (!counters) {
+    own.alloc(NBIN);
+    counters = own.p;
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, counters, (i64)NBIN, 0);
   const dim3 grid((unsigned)((n + 255) / 256));
   if (n > 0) {
     ProfScope ps("bin_count", s);
-    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, dcnt.p);
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, counters);
   }
   RT_HIP(hipGetLastError());
-  RT_HIP(hipMemcpyAsync(cnt, dcnt.p, NBIN * sizeof(i32), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipMemcpyAsync(cnt, counters, NBIN * sizeof(i32), hipMemcpyDeviceToHost, s));
   RT_HIP(hipStreamSynchronize(s));
   off[0] = 0;
   for (int b = 0; b < NBIN; ++b) off[b + 1] = off[b] + cnt[b];
-  RT_HIP(hipMemcpyAsync(dcnt.p, off, NBIN * sizeof(i32), hipMemcpyHostToDevice, s));
+  RT_HIP(hipMemcpyAsync(counters, off, NBIN * sizeof(i32), hipMemcpyHostToDevice, s));
   if (n > 0) {
     ProfScope ps("bin_fill", s);
-    hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, dcnt.p, list);
+    hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, counters, list);
   }
   RT_HIP(hipGetLastError());
   RT_HIP(hipStreamSynchronize(s));
@@ -2621,6 +2639,149 @@ void spgemm_plain_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& 
     plain_graph_impl<true>(A, B, batch.b_base, C, tune, s, st);
   else
     plain_graph_impl<false>(A, B, nullptr, C, tune, s, st);
+}
+
+}  // namespace rt
+
+// ---------------------------------------------------------------------------
+// The exact two-phase product without any device allocation
+// (rt_spmspm_noalloc), for callers that cannot allocate: C goes into the
+// caller's arrays, and every scratch buffer of the pipeline -- per-row
+// products and counts, the two row-bin lists and their counters, the scan
+// partials, the validation flag, B's tile boundaries, the long rows' tile
+// counts and the row-slot counters -- is carved from `scratch`, one
+// file-scope device array of NOALLOC_SCRATCH_BYTES.  The kernels are those of
+// the direct exact method (LDS-hash bins, LDS-bitmap symbolic and dense-tile
+// numeric long-row kernels), launched with host-read bin counts.
+namespace rt {
+
+// In host code this name is only a stand-in for the device symbol: kernels
+// get the address hipGetSymbolAddress returns (scratch_base).
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Device id -> that device's copy of `scratch`, looked up once per device
+// (with the device current) on the first call there.
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_addr;
+
+unsigned char* scratch_base() {
+  int dev_id = 0;
+  RT_HIP(hipGetDevice(&dev_id));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_addr.find(dev_id);
+  if (it != g_scratch_addr.end()) return it->second;
+  void* p = nullptr;
+  RT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  RT_CHECK(p != nullptr, RT_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_addr[dev_id] = static_cast<unsigned char*>(p);
+  return static_cast<unsigned char*>(p);
+}
+
+// Bump allocation of 256-byte aligned pieces of the scratch array.
+struct ScratchCarve {
+  unsigned char* base;
+  i64 used = 0;
+  i64 need(i64 bytes) const { return ((used + 255) & ~(i64)255) + bytes; }
+  template <class T>
+  T* take(i64 count) {
+    const i64 off = (used + 255) & ~(i64)255;
+    const i64 bytes = std::max<i64>(count, 1) * (i64)sizeof(T);
+    RT_CHECK(off + bytes <= NOALLOC_SCRATCH_BYTES, RT_E_LIMIT,
+             "rt_spmspm_noalloc: this product needs more than the " +
+                 std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                 " MiB of scratch the entry point has (at least " +
+                 std::to_string((off + bytes + (1 << 20) - 1) >> 20) +
+                 " MiB); use rt_spmspm_ex");
+    used = off + bytes;
+    return reinterpret_cast<T*>(base + off);
+  }
+};
+
+// Calls on one device share its scratch: they are serialised here (each
+// call is synchronous, so the next one starts after the previous finished).
+std::mutex g_noalloc_mu;
+
+}  // namespace
+
+void spgemm_noalloc(const Csr& A, const Csr& B, i64* crp, i32* ccol, float* cval, i64 capacity,
+                    i64* c_nnz, bool trusted, hipStream_t s) {
+  std::lock_guard<std::mutex> serial(g_noalloc_mu);
+  if (c_nnz) *c_nnz = -1;
+  const i64 m = A.rows, M = std::max<i64>(m, 1);
+  RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
+  const i32 nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
+  const Tuning tu;
+  ScratchCarve sc{scratch_base()};
+  // Everything but the long rows' pieces is taken (and its size checked)
+  // before the first launch.
+  int* flag = sc.take<int>(1);
+  unsigned long long* tot = sc.take<unsigned long long>(2);
+  i32* longq = sc.take<i32>((i64)row_products_queue(A.nnz));
+  i64* prod = sc.take<i64>(M);
+  i64* nnz = sc.take<i64>(M);
+  i32* slist = sc.take<i32>(M);
+  i32* nlist = sc.take<i32>(M);
+  i32* counters = sc.take<i32>(NBIN);
+  i64* partials = sc.take<i64>(scan_partials(m));
+  i32* next = sc.take<i32>(2);
+  if (!trusted) {
+    validate_csr_dev(A, A.cols, nullptr, s, "A", flag);
+    validate_csr_dev(B, B.cols, nullptr, s, "B", flag);
+  }
+  // Symbolic phase.
+  hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(tot), (i64)2,
+                     (i64)0);
+  launch_row_products<false>(m, A.rp, A.col, B.rp, nullptr, prod, tot, longq, s);
+  hipLaunchKernelGGL(k_fill_i64, dim3(fill_grid(M)), dim3(256), 0, s, nnz, M, (i64)0);
+  RT_HIP(hipGetLastError());
+  int scnt[NBIN], soff[NBIN + 1];
+  bin_rows(prod, m, B.cols, tu.sym, slist, scnt, soff, s, counters);
+  HashArgs a = make_args(A, B, nullptr);
+  a.nnz = nnz;
+  launch_hash_bins<MODE_SYM, false>(a, slist, scnt, soff, tu.shape, s);
+  const i32 nlong = scnt[5];
+  i32* slab = nullptr;
+  i32* tilecnt = nullptr;
+  if (nlong > 0) {
+    slab = sc.take<i32>(std::max<i64>(B.rows, 1) * (nt + 1));
+    tilecnt = sc.take<i32>((i64)nlong * nt);
+    if (B.rows > 0)
+      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)),
+                         dim3(256), 0, s, B.rows, nt, B.rp, B.col, slab);
+    zero_slot(next, s);
+    const RowSet rs{slist + soff[5], nlong, nullptr, nullptr, 0};
+    launch_sym_bitmap<false>((unsigned)std::min<i64>(nlong, (i64)device().cus * 2), rs, A, B,
+                             nullptr, slab, nt, tilecnt, nnz, nullptr, nullptr, next, s);
+    RT_HIP(hipGetLastError());
+  }
+  exclusive_scan_i64(nnz, crp, m, s, partials);
+  const i64 total = read_scalar(crp + m, s);
+  if (c_nnz) *c_nnz = total;
+  RT_CHECK(total <= capacity, RT_E_LIMIT,
+           "rt_spmspm_noalloc: C has " + std::to_string(total) + " entries, the output arrays hold " +
+               std::to_string(capacity) + " (C's row pointer is filled)");
+  RT_CHECK(total == 0 || (ccol && cval), RT_E_INVALID, "rt_spmspm_noalloc: NULL C arrays");
+  // Numeric phase.
+  int ncnt[NBIN], noff[NBIN + 1];
+  bin_rows(nnz, m, LLONG_MAX, tu.num, nlist, ncnt, noff, s, counters);
+  if (ncnt[5] > 0) {
+    zero_slot(next + 1, s);
+    const RowSet rs{slist + soff[5], nlong, nullptr, nullptr, 0};
+    hipLaunchKernelGGL((k_num_heavy<false, false>),
+                       dim3((unsigned)std::min<i64>(nlong, (i64)device().cus * 2)), dim3(HWG), 0,
+                       s, rs, nt, tilecnt, nnz, tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val,
+                       nullptr, slab, crp, ccol, cval, nullptr, nullptr, nullptr, next + 1);
+    RT_HIP(hipGetLastError());
+  }
+  a = make_args(A, B, nullptr);
+  a.crp = crp;
+  a.ccol = ccol;
+  a.cval = cval;
+  launch_hash_bins<MODE_NUM, false>(a, nlist, ncnt, noff, tu.shape, s);
+  RT_HIP(hipGetLastError());
+  RT_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace rt

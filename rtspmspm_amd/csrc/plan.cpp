@@ -20,11 +20,6 @@ struct rt_plan {
   float* c_val = nullptr;  // identifies the C this plan filled
 };
 
-namespace rt {
-void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
-                      const char* what);
-}
-
 using namespace rt;
 
 namespace {

@@ -86,8 +86,9 @@ struct HeavyRows {
 // exact nnz; learned keys are predicted capacities.
 void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipStream_t s,
                   RunStats* st);
+// counters: NBIN device ints of scratch (nullptr: taken from the pool).
 void bin_rows(const i64* x, i64 n, i64 cap, const i64 th[5], i32* list, int cnt[6], int off[7],
-              hipStream_t s);
+              hipStream_t s, i32* counters = nullptr);
 // Reorders the long-row list `rows` (n entries) so that rows gathering the
 // same long B rows run together (k_order.hip).
 void order_long_rows(const Csr& A, const Csr& B, i32* rows, i32 n, hipStream_t s);
@@ -127,6 +128,19 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
 void row_features(const Csr& A, const Csr& B, const Batch& batch, float* feat,
                   const i64* prod, hipStream_t s);
 
+// Structural check of a device CSR (k_validate); RT_E_INVALID when
+// malformed.  flag: one device int of scratch (nullptr: from the pool).
+void validate_csr_dev(const Csr& M, i64 col_limit, const i64* row_limit, hipStream_t s,
+                      const char* what, int* flag = nullptr);
+
+// C = A*B into caller-provided arrays, every scratch buffer taken from one
+// file-scope device array (rt_spmspm_noalloc).  *c_nnz receives nnz(C) (or
+// -1 when the scratch is too small); RT_E_LIMIT when C needs more than
+// `capacity` entries (c_row_ptr is then filled, c_col / c_val untouched).
+constexpr i64 NOALLOC_SCRATCH_BYTES = (i64)64 << 20;
+void spgemm_noalloc(const Csr& A, const Csr& B, i64* c_row_ptr, i32* c_col, float* c_val,
+                    i64 capacity, i64* c_nnz, bool trusted, hipStream_t s);
+
 // Fingerprint of a result (k_fingerprint): structure hashes and value sums.
 struct Fingerprint {
   i64 nnz = 0;
@@ -136,7 +150,10 @@ struct Fingerprint {
 Fingerprint fingerprint(const CsrOut& C, hipStream_t s);
 
 // Device utilities.
-void exclusive_scan_i64(const i64* in, i64* out /* n+1 */, i64 n, hipStream_t s);
+// partials: scan_partials(n) device i64 of scratch (nullptr: from the pool).
+void exclusive_scan_i64(const i64* in, i64* out /* n+1 */, i64 n, hipStream_t s,
+                        i64* partials = nullptr);
+i64 scan_partials(i64 n);
 // out[r] = values[b] for every row r of segment b (segments of seg_rows[b]
 // consecutive rows); used for the per-row B base and column limits of batches.
 void fill_segments(i32 count, const i64* seg_rows, const i64* values, i64* out_dev,
