@@ -10,9 +10,17 @@ flops = 2P; value = 2P / t in GFLOP/s.  Algorithmic bytes of the whole
 product (A read once, C written once, CSR with int64 row pointers and
 4+4-byte entries): 8(m+1) + 8 nnz(A) + 8(m+1) + 8 nnz(C).
 
-Before timing, the library's autotuner picks the row-bin thresholds for this
+Before timing, the library's autotuner picks the row-bin settings for this
 matrix class for each method timed (untimed, cached in a temporary tuning
-file); both methods then run with their tuned settings.
+file); both methods then run with their tuned settings.  The operands are
+checked once (an untimed call); the timed calls pass them as trusted.
+
+The same line carries, under config.workloads, the other full-size
+workloads, each tuned and timed the same way: C = A*A of a 4,096 x 4,096
+matrix with 8 uniform columns per row (plain, learned, ray); the Galerkin
+triple product R*(A*P) on a 128^3 grid; the rectangular 500,000 x 2,000,000
+times 2,000,000 x 500,000 product; and the batch of 2,048 independent
+products (batched plain and learned against 2,048 sequential plain calls).
 
 Prints ONE JSON line on stdout (human notes go to stderr).
 """
@@ -78,6 +86,17 @@ def kernel_bytes(name, A, P, cnnz, ncols, TH=TH):
     return None
 
 
+def host_threads():
+    """Host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (on a shared GPU box it is the CPU share
+    of one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def cpu_baseline(A, P, threads, target_s=15.0):
     """FP32 Gustavson (reference semantics) of a strided row sample of the
     same product, on `threads` host cores; returns GFLOP/s and the sample."""
@@ -136,105 +155,185 @@ def timed_calls(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps * 1e3, last
 
 
-def run_workload(args):
-    """The workloads other than the headline one: one JSON line with plain and
-    learned (and ray for uniform4k, sequential-vs-batched for batch2048)."""
+def _timed_product(call, steps, warmup):
+    """Warm-up, then `steps` timed calls of call() -> (C, info, free).
+    Returns (wall ms per call, median device ms, last info, last C's nnz)."""
+    import rtspmspm_amd as rt
+    for _ in range(warmup):
+        C, info, free = call()
+        free()
+    rt.synchronize()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        C, info, free = call()
+        dev.append(info["ms"])
+        nnz = C.nnz
+        free()
+    rt.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, float(np.median(dev)), info, nnz
+
+
+def _entry(ms, dev_ms, flops, bytes_alg, info=None):
+    e = {"ms": round(ms, 4), "device_ms": round(dev_ms, 4), "gflops": round(flops / ms / 1e6, 3),
+         "roofline_frac": round(bytes_alg / ms / 1e6 / HBM_PEAK_GBS, 6)}
+    if info is not None and info.get("predicted_rows"):
+        e["predicted_rows"] = info["predicted_rows"]
+        e["overflow_rows"] = info["overflow_rows"]
+    return e
+
+
+def _speedups(out, base="plain"):
+    for k, v in out.items():
+        if isinstance(v, dict) and "ms" in v and k != base and base in out:
+            v["speedup_vs_" + base] = round(out[base]["ms"] / v["ms"], 4)
+
+
+def wl_single(name, A, B, desc, steps, warmup, tune, modes=("plain", "learned")):
+    """A single product C = A*B: each mode tuned (ray: untuned) and timed."""
     import rtspmspm_amd as rt
     from rtspmspm_amd import workloads as W
-    out = {"workload": args.workload, "steps": args.steps, "warmup": args.warmup}
+    P = int(W.row_products(A, B).sum())
+    dA = rt.to_device(A)
+    dB = dA if B is A else rt.to_device(B)
+    C0, _ = rt.spmspm(dA, dB)  # operands checked once; the timed calls trust them
+    nnz = C0.nnz
+    C0.free()
+    bytes_alg = (W.csr_bytes(A.shape[0], A.nnz) + (0 if B is A else W.csr_bytes(B.shape[0], B.nnz))
+                 + W.csr_bytes(A.shape[0], nnz))
+    out = {"config": desc, "products": P, "nnz_C": nnz, "bytes_alg": bytes_alg, "tuning": {}}
+    for mode in modes:
+        if tune and mode != "ray":
+            out["tuning"][mode] = rt.autotune(dA, dB, mode=mode, reps=1).split("; ", 1)[-1]
+
+        def call(mode=mode):
+            C, info = rt.spmspm(dA, dB, mode=mode, use_tuned=tune, trusted=True)
+            return C, info, C.free
+        ms, dev_ms, info, got = _timed_product(call, steps, warmup)
+        assert got == nnz, f"{name} {mode}: nnz(C) {got} != {nnz}"
+        out[mode] = _entry(ms, dev_ms, 2 * P, bytes_alg, info)
+        log(f"  {name} {mode}: {ms:.3f} ms ({dev_ms:.3f} device), {2 * P / ms / 1e6:.1f} GFLOP/s")
+    _speedups(out)
+    rt.pool_trim()
+    return out
+
+
+def wl_galerkin(grid, steps, warmup, tune):
+    """R*(A*P) on a grid^3 27-point stencil with 2x2x2 aggregation."""
+    import rtspmspm_amd as rt
+    from rtspmspm_amd import workloads as W
+    A, Pm, R = W.galerkin(grid)
+    dA, dP, dR = rt.to_device(A), rt.to_device(Pm), rt.to_device(R)
+    AP0, _ = rt.spmspm(dA, dP)
+    C0, i2 = rt.spmspm(dR, AP0)
+    P1, P2 = int(W.row_products(A, Pm).sum()), i2["products"]
+    nnz_ap, nnz_c = AP0.nnz, C0.nnz
+    out = {"config": f"R*(A*P), 27-point stencil on {grid}^3, 2x2x2 aggregation",
+           "products": P1 + P2, "nnz_AP": nnz_ap, "nnz_C": nnz_c, "tuning": {}}
+    out["bytes_alg"] = (W.csr_bytes(A.shape[0], A.nnz) + W.csr_bytes(Pm.shape[0], Pm.nnz)
+                        + 2 * W.csr_bytes(A.shape[0], nnz_ap) + W.csr_bytes(R.shape[0], R.nnz)
+                        + W.csr_bytes(R.shape[0], nnz_c))
+    for mode in ("plain", "learned"):
+        if tune:
+            out["tuning"][mode] = [rt.autotune(dA, dP, mode=mode, reps=1).split("; ", 1)[-1],
+                                   rt.autotune(dR, AP0, mode=mode, reps=1).split("; ", 1)[-1]]
+
+        def call(mode=mode):
+            AP, i1 = rt.spmspm(dA, dP, mode=mode, use_tuned=tune, trusted=True)
+            C, i2 = rt.spmspm(dR, AP, mode=mode, use_tuned=tune, trusted=True)
+            AP.free()
+            info = {"ms": i1["ms"] + i2["ms"],
+                    "predicted_rows": i1["predicted_rows"] + i2["predicted_rows"],
+                    "overflow_rows": i1["overflow_rows"] + i2["overflow_rows"]}
+            return C, info, C.free
+        ms, dev_ms, info, got = _timed_product(call, steps, warmup)
+        assert got == nnz_c
+        out[mode] = _entry(ms, dev_ms, 2 * (P1 + P2), out["bytes_alg"], info)
+        log(f"  galerkin {mode}: {ms:.3f} ms ({dev_ms:.3f} device)")
+    AP0.free()
+    C0.free()
+    _speedups(out)
+    rt.pool_trim()
+    return out
+
+
+def wl_batch(count, steps, warmup, tune):
+    """`count` independent products in one batched call (plain and learned)
+    against `count` sequential plain calls."""
+    import rtspmspm_amd as rt
+    from rtspmspm_amd import workloads as W
     t0 = time.perf_counter()
+    ar, br, bc, SA, SB = W.batch(count)
+    dSA, dSB = rt.to_device(SA), rt.to_device(SB)
+    dpairs = [(rt.to_device(a), rt.to_device(b)) for a, b in W.split_pairs(ar, br, bc, SA, SB)]
+    log(f"  batch {count}: operands ready in {time.perf_counter() - t0:.1f} s, "
+        f"rows {int(ar.sum())}, nnz(A) {SA.nnz}")
+    C0, i0 = rt.spmspm_batched(ar, br, bc, dSA, dSB)
+    P, nnz_c = i0["products"], C0.nnz
+    C0.free()
+    bytes_alg = (W.csr_bytes(SA.shape[0], SA.nnz) + W.csr_bytes(SB.shape[0], SB.nnz)
+                 + W.csr_bytes(SA.shape[0], nnz_c))
+    out = {"config": f"{count} pairs, n log-uniform [256, 65536], Poisson(U(4,32)) per row, "
+                     "seed 4", "products": P, "nnz_C": nnz_c, "bytes_alg": bytes_alg, "tuning": {}}
+    for mode in ("plain", "learned"):
+        if tune:
+            out["tuning"]["batched_" + mode] = rt.autotune_batched(
+                ar, br, bc, dSA, dSB, mode=mode, reps=1).split("; ", 1)[-1]
 
-    def one(dA, dB, mode):
-        def f():
-            C, info = rt.spmspm(dA, dB, mode=mode)
-            return (C, info), C.free
-        return f
+        def call(mode=mode):
+            C, info = rt.spmspm_batched(ar, br, bc, dSA, dSB, mode=mode, use_tuned=tune,
+                                        trusted=True)
+            return C, info, C.free
+        ms, dev_ms, info, got = _timed_product(call, steps, warmup)
+        assert got == nnz_c
+        out["batched_" + mode] = _entry(ms, dev_ms, 2 * P, bytes_alg, info)
+        log(f"  batched {mode}: {ms:.2f} ms ({dev_ms:.2f} device), {2 * P / ms / 1e6:.1f} GFLOP/s")
 
-    if args.workload in ("uniform4k", "rect"):
-        if args.workload == "uniform4k":
+    def seq():
+        res = [rt.spmspm(a, b, mode="plain", trusted=True)[0] for a, b in dpairs]
+        n = sum(c.nnz for c in res)
+
+        class _All:
+            nnz = n
+        return _All, {"ms": 0.0}, (lambda: [c.free() for c in res])
+    ms, _, _, got = _timed_product(seq, max(1, steps // 2), 1)
+    assert got == nnz_c, "batched and sequential results differ in size"
+    out["sequential_plain"] = _entry(ms, 0.0, 2 * P, bytes_alg)
+    del out["sequential_plain"]["device_ms"]
+    for k in ("batched_plain", "batched_learned"):
+        out[k]["speedup_vs_sequential"] = round(ms / out[k]["ms"], 4)
+    out["batched_learned"]["speedup_vs_batched_plain"] = round(
+        out["batched_plain"]["ms"] / out["batched_learned"]["ms"], 4)
+    log(f"  sequential plain ({count} calls): {ms:.2f} ms -> batched learned "
+        f"{out['batched_learned']['speedup_vs_sequential']:.2f}x")
+    rt.pool_trim()
+    return out
+
+
+def run_workloads(names, args, tune):
+    from rtspmspm_amd import workloads as W
+    res = {}
+    for name in names:
+        t0 = time.perf_counter()
+        if name == "uniform4k":
             A = W.uniform4k()
-            B = A
-            desc = "A*A, A 4096 x 4096, 8 uniform columns per row, seed 0"
-        else:
+            res[name] = wl_single(name, A, A, "A*A, A 4096 x 4096, 8 uniform columns per row, "
+                                  "U(-1,1), seed 0", args.steps, args.warmup, tune,
+                                  ("plain", "learned", "ray"))
+        elif name == "galerkin":
+            res[name] = wl_galerkin(args.grid, args.steps, args.warmup, tune)
+        elif name == "rect":
             A, B = W.rect()
-            desc = "A 500000 x 2000000 lognormal rows * B 2000000 x 500000 4 per row, seed 3"
-        P = int(W.row_products(A, B).sum())
-        dA, dB = rt.to_device(A), rt.to_device(B)
-        log(f"{args.workload}: generated in {time.perf_counter() - t0:.1f} s, products {P}")
-        modes = ["plain", "learned"] + (["ray"] if args.workload == "uniform4k" else [])
-        for mode in modes:
-            ms, ((C, info), free) = timed_calls(one(dA, dB, mode), args.steps, args.warmup)
-            nnz = C.nnz
-            free()
-            out[mode] = {"ms": round(ms, 4), "gflops": round(2 * P / ms / 1e6, 3),
-                         "overflow_rows": info["overflow_rows"],
-                         "predicted_rows": info["predicted_rows"]}
-            log(f"  {mode}: {ms:.3f} ms, {2 * P / ms / 1e6:.2f} GFLOP/s, nnz(C) {nnz}")
-        bytes_alg = W.csr_bytes(A.shape[0], A.nnz) + (0 if B is A else W.csr_bytes(B.shape[0], B.nnz)) \
-            + W.csr_bytes(A.shape[0], nnz)
-        out.update(config=desc, products=P, nnz_C=nnz, bytes_alg=bytes_alg)
-    elif args.workload == "galerkin":
-        A, Pm, R = W.galerkin(args.grid)
-        dA, dP, dR = rt.to_device(A), rt.to_device(Pm), rt.to_device(R)
-        P1 = int(W.row_products(A, Pm).sum())
-        log(f"galerkin {args.grid}^3: generated in {time.perf_counter() - t0:.1f} s")
-        for mode in ("plain", "learned"):
-            def chain():
-                AP, i1 = rt.spmspm(dA, dP, mode=mode)
-                C, i2 = rt.spmspm(dR, AP, mode=mode)
-                nnz_ap = AP.nnz
-                AP.free()
-                return (C, nnz_ap, i1, i2), C.free
-            ms, ((C, nnz_ap, i1, i2), free) = timed_calls(chain, args.steps, args.warmup)
-            nnz_c = C.nnz
-            free()
-            P2 = i2["products"]
-            out[mode] = {"ms": round(ms, 4), "gflops": round(2 * (P1 + P2) / ms / 1e6, 3),
-                         "overflow_rows": i1["overflow_rows"] + i2["overflow_rows"]}
-            log(f"  {mode}: {ms:.3f} ms, {2 * (P1 + P2) / ms / 1e6:.2f} GFLOP/s, "
-                f"nnz(AP) {nnz_ap}, nnz(C) {nnz_c}")
-        bytes_alg = (W.csr_bytes(A.shape[0], A.nnz) + W.csr_bytes(Pm.shape[0], Pm.nnz)
-                     + 2 * W.csr_bytes(A.shape[0], nnz_ap) + W.csr_bytes(R.shape[0], R.nnz)
-                     + W.csr_bytes(R.shape[0], nnz_c))
-        out.update(config=f"R*(A*P), 27-point stencil on {args.grid}^3, 2x2x2 aggregation",
-                   products=P1 + P2, nnz_AP=nnz_ap, nnz_C=nnz_c, bytes_alg=bytes_alg)
-    elif args.workload == "batch2048":
-        ar, br, bc, SA, SB = W.batch(args.count)
-        dSA, dSB = rt.to_device(SA), rt.to_device(SB)
-        pairs = W.split_pairs(ar, br, bc, SA, SB)
-        dpairs = [(rt.to_device(a), rt.to_device(b)) for a, b in pairs]
-        log(f"batch {args.count}: generated in {time.perf_counter() - t0:.1f} s, "
-            f"rows {int(ar.sum())}, nnz(A) {SA.nnz}")
-        for mode in ("plain", "learned"):
-            def batched():
-                C, info = rt.spmspm_batched(ar, br, bc, dSA, dSB, mode=mode)
-                return (C, info), C.free
-            ms, ((C, info), free) = timed_calls(batched, args.steps, args.warmup)
-            nnz_c, P = C.nnz, info["products"]
-            free()
-            out["batched_" + mode] = {"ms": round(ms, 3), "gflops": round(2 * P / ms / 1e6, 3),
-                                      "overflow_rows": info["overflow_rows"]}
-            log(f"  batched {mode}: {ms:.2f} ms, {2 * P / ms / 1e6:.1f} GFLOP/s, nnz(C) {nnz_c}")
+            res[name] = wl_single(name, A, B, "A 500000 x 2000000 lognormal(2.5,1) rows * "
+                                  "B 2000000 x 500000 4 per row, N(0,1), seed 3",
+                                  args.steps, args.warmup, tune)
+        elif name == "batch2048":
+            res[name] = wl_batch(args.count, args.steps, args.warmup, tune)
+        res[name]["seconds"] = round(time.perf_counter() - t0, 1)
+    return res
 
-        def sequential():
-            res = [rt.spmspm(a, b, mode="plain")[0] for a, b in dpairs]
-            return res, (lambda: [c.free() for c in res])
-        ms_seq, (res, free) = timed_calls(sequential, max(1, args.steps // 2), 1)
-        seq_nnz = sum(c.nnz for c in res)
-        free()
-        assert seq_nnz == nnz_c, "batched and sequential results differ in size"
-        out["sequential_plain"] = {"ms": round(ms_seq, 3), "gflops": round(2 * P / ms_seq / 1e6, 3)}
-        out["batched_speedup_vs_sequential"] = round(ms_seq / out["batched_learned"]["ms"], 3)
-        log(f"  sequential plain ({args.count} launches): {ms_seq:.2f} ms -> batched learned "
-            f"{out['batched_speedup_vs_sequential']:.2f}x")
-        bytes_alg = (W.csr_bytes(SA.shape[0], SA.nnz) + W.csr_bytes(SB.shape[0], SB.nnz)
-                     + W.csr_bytes(SA.shape[0], nnz_c))
-        out.update(config=f"{args.count} pairs, n log-uniform [256, 65536], Poisson(U(4,32)), seed 4",
-                   products=P, nnz_C=nnz_c, bytes_alg=bytes_alg)
-    for k in ("plain", "learned", "ray", "batched_plain", "batched_learned"):
-        if k in out:
-            out[k]["roofline_frac"] = round(out["bytes_alg"] / out[k]["ms"] / 1e6 / HBM_PEAK_GBS, 6)
-    print(json.dumps(out), flush=True)
+
+WORKLOADS = ("uniform4k", "galerkin", "rect", "batch2048")
 
 
 def main():
@@ -256,6 +355,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
     ap.add_argument("--no-tune", action="store_true",
                     help="run with the library's default row-bin thresholds")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="leave the other workloads out of the headline line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library's device_copy kernel over BYTES (counter "
                          "calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -273,14 +374,15 @@ def main():
         sys.exit(2)
     if "RANK" in os.environ and int(os.environ.get("WORLD_SIZE", "1")) > 1:
         sys.exit("bench.py is single-GPU")
-    if args.workload != "rmat20":
-        run_workload(args)
-        return
-
     if not args.no_tune:
         import tempfile
         os.environ.setdefault("RTSPMSPM_TUNE_CACHE",
                               os.path.join(tempfile.gettempdir(), f"rtspmspm_bench_{os.getpid()}.txt"))
+    if args.workload != "rmat20":
+        out = run_workloads([args.workload], args, not args.no_tune)[args.workload]
+        print(json.dumps({"workload": args.workload, "steps": args.steps,
+                          "warmup": args.warmup, **out}), flush=True)
+        return
     import rtspmspm_amd as rt
     dev = rt.device_info()
     log(f"device {dev['name']}, {dev['cus']} CUs, {dev['total_mem'] / 2**30:.0f} GiB")
@@ -291,6 +393,8 @@ def main():
     log(f"R-MAT s{args.scale}: n={A.shape[0]} nnz={A.nnz} products={ptot} "
         f"(gen {time.perf_counter() - t0:.1f} s)")
     dA = rt.to_device(A)
+    C0, _ = rt.spmspm(dA, dA)  # operands checked once; the timed calls trust them
+    C0.free()
     tuned = not args.no_tune
     tuning = {}
     th_used = TH
@@ -309,13 +413,13 @@ def main():
         """Warm-up, then K timed steps; returns (wall ms/step, device ms list,
         last C's per-row nnz, last info)."""
         for _ in range(args.warmup):
-            C, info = rt.spmspm(dA, dA, mode=method, use_tuned=tuned)
+            C, info = rt.spmspm(dA, dA, mode=method, use_tuned=tuned, trusted=True)
             C.free()
         rt.synchronize()
         ms_dev = []
         t0 = time.perf_counter()
         for s in range(args.steps):
-            C, info = rt.spmspm(dA, dA, mode=method, use_tuned=tuned)
+            C, info = rt.spmspm(dA, dA, mode=method, use_tuned=tuned, trusted=True)
             ms_dev.append(info["ms"])
             if s + 1 < args.steps:
                 C.free()
@@ -350,7 +454,7 @@ def main():
     kernels = {}
     if not args.no_profile:
         rt.prof_reset()
-        C, _ = rt.spmspm(dA, dA, mode=args.method, profile=True, use_tuned=tuned)
+        C, _ = rt.spmspm(dA, dA, mode=args.method, profile=True, use_tuned=tuned, trusted=True)
         C.free()
         stats = rt.prof_stats()
         kernels = {k: {"launches": v[0], "ms": round(v[1], 4)} for k, v in stats.items()}
@@ -373,9 +477,22 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(A, P, threads)
-        log(f"cpu baseline: {cpu['value']:.2f} GFLOP/s on {threads} threads ({cpu['sample']})")
+        threads = host_threads()
+        # 1 thread first: the oracle sets OpenMP's thread count for the
+        # process, and the many-thread run restores it.
+        one = cpu_baseline(A, P, 1, target_s=6.0)
+        cpu = cpu_baseline(A, P, threads, target_s=12.0)
+        cpu["nproc"] = os.cpu_count()
+        cpu["value_1thread"] = one["value"]
+        cpu["sample_1thread"] = one["sample"]
+        log(f"cpu baseline: {cpu['value']:.2f} GFLOP/s on {threads} threads of {os.cpu_count()} "
+            f"({cpu['sample']}); 1 thread {one['value']:.3f} GFLOP/s ({one['sample']})")
+
+    workloads = None
+    if not args.no_workloads:
+        del dA
+        rt.pool_trim()
+        workloads = run_workloads(WORKLOADS, args, tuned)
 
     out = {
         "metric": METRIC, "value": round(gflops, 3), "unit": "GFLOP/s", "n_gpus": 1,
@@ -390,8 +507,9 @@ def main():
                    "pipeline_roofline_frac": round(bytes_alg / ms_per_step / 1e6 / HBM_PEAK_GBS, 5),
                    "device_ms_median": round(float(np.median(ms_dev)), 3),
                    "tuned": tuned, "num_thresholds": th_used,
-                   "tuning": {k: v.split("; thresholds ")[-1] for k, v in tuning.items()},
-                   **compare},
+                   "tuning": {k: v.split("; ", 1)[-1] for k, v in tuning.items()},
+                   "timed_calls": "operands checked once before timing, then passed as trusted",
+                   **compare, "workloads": workloads},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -138,7 +138,9 @@ double timed_product(const Csr& A, const Csr& B, const Batch& bt, int mode, cons
   EventPair ev;
   RT_HIP(hipEventRecord(ev.a, s));
   try {
-    if (mode == RT_MODE_LEARNED) spgemm_learned(A, B, bt, C, tu, s, &st);
+    // batched calls run their stages as graphs, as rt_spmspm_batched does
+    if (mode == RT_MODE_LEARNED && bt.b_base) spgemm_learned_graph(A, B, bt, C, tu, s, &st);
+    else if (mode == RT_MODE_LEARNED) spgemm_learned(A, B, bt, C, tu, s, &st);
     else if (bt.b_base) spgemm_plain_graph(A, B, bt, C, tu, s, &st);
     else spgemm_plain(A, B, bt, C, tu, s, &st);
   } catch (...) {
@@ -175,7 +177,8 @@ void run_device(const Csr& A, const Csr& B, const Batch& bt, CsrOut& C, rt_opts&
         spgemm_ray(A, B, C, o.ablation, o.ray_output, s, &st);
         break;
       case RT_MODE_LEARNED:
-        spgemm_learned(A, B, bt, C, tu, s, &st);
+        if (graph) spgemm_learned_graph(A, B, bt, C, tu, s, &st);
+        else spgemm_learned(A, B, bt, C, tu, s, &st);
         break;
       default: fail(RT_E_INVALID, "unknown mode " + std::to_string(o.mode));
     }

@@ -470,7 +470,8 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
   if (tu.learned_bins == 0 && !tu.learned_heavy) {
     // The tuning predicts no row: every row takes the exact path.
-    spgemm_plain(A, B, batch, C, tu, s, st);
+    if (batch.b_base) spgemm_plain_graph(A, B, batch, C, tu, s, st);
+    else spgemm_plain(A, B, batch, C, tu, s, st);
     return;
   }
   std::vector<float> hp;
@@ -544,6 +545,49 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
 void row_features(const Csr& A, const Csr& B, const Batch& batch, float* feat, const i64* prod,
                   hipStream_t s) {
   features_dev(A, B, batch.b_base, prod, feat, s);
+}
+
+// ---- launch wrappers for the graph-captured learned path ----------------------
+// Kernels only: no allocation, no host synchronisation, no memset, so that a
+// stream capture records them (spgemm_learned_graph, k_spgemm.hip).
+void learned_weights(std::vector<float>& params, float& margin) { current_weights(params, margin); }
+
+void learned_predict_launch(const Csr& A, const Csr& B, const i64* bbase, const i64* prod,
+                            const float* params, float* feat, float* pred, i64 skip_u,
+                            hipStream_t s) {
+  features_dev(A, B, bbase, prod, feat, s, skip_u);
+  predict_dev(params, feat, A.rows, pred, s);
+}
+
+void learned_capacity_launch(i64 m, const i64* prod, i64 ncols, const float* pred, float margin,
+                             const Tuning& tu, i64* cap, i64* key_light, i64* key_heavy,
+                             i64* key_exact, i32* ovf, hipStream_t s) {
+  if (m <= 0) return;
+  BinTop bt;
+  for (int b = 0; b < 5; ++b) bt.t[b] = tu.num[b];
+  hipLaunchKernelGGL(k_capacity, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, prod,
+                     ncols, pred, margin, bt, std::min(tu.sym[4], tu.num[4]), tu.learned_heavy,
+                     tu.learned_bins, cap, key_light, key_heavy, key_exact, ovf);
+  RT_HIP(hipGetLastError());
+}
+
+void learned_merge_launch(i64 m, const i64* prod, const i64* cap, const i32* ovf, i64* key_exact,
+                          unsigned long long* counts, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_merge_overflow, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, prod,
+                     cap, ovf, key_exact, counts);
+  RT_HIP(hipGetLastError());
+}
+
+void learned_finish_launch(i64 m, const i64* cap, const i32* ovf, const i64* tmp_off,
+                           const i32* tcol, const float* tval, const i64* key_exact,
+                           const i64* nnz, i64* key_num, const CsrOut& C, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * WAVE + 255) / 256)), dim3(256), 0, s, m, cap,
+                     ovf, tmp_off, tcol, tval, C.rp, C.col, C.val);
+  hipLaunchKernelGGL(k_numeric_keys, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m,
+                     key_exact, nnz, key_num);
+  RT_HIP(hipGetLastError());
 }
 
 // ---- training ------------------------------------------------------------------

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "learned.hpp"
 #include "spgemm.hpp"
 
 namespace rt {
@@ -1909,9 +1910,10 @@ void exclusive_scan_i64(const i64* in, i64* out, i64 n, hipStream_t s, i64* part
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(1024), 0, s, in, n, partials, out);
   }
   RT_HIP(hipGetLastError());
-  // bsum is released to the pool here; the pool only hands it out again to
-  // work ordered after this on the same stream or after a synchronisation.
-  RT_HIP(hipStreamSynchronize(s));
+  // Partials of our own are released to the pool here; the pool only hands
+  // them out again after this synchronisation.  Caller partials need none
+  // (and a captured stream allows none).
+  if (own.p) RT_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
@@ -2513,27 +2515,58 @@ void launch_hash_bins_dev(const HashArgs& a, const DevBins& b, i64 max_rows, con
   for_hash_bins<MODE, BATCH>(shape, l);
 }
 
-// Captures the launches `body` issues on `s` into a graph, runs it once and
-// returns the device time of the run.
+// Instantiated graphs of this host thread, by stage tag.  A stage captured
+// again with the same tag is written into the cached executable graph with
+// hipGraphExecUpdate (new buffer addresses and launch sizes) instead of
+// being instantiated anew; when the update is refused (another topology) the
+// graph is instantiated and replaces the cached one.  Kept until the
+// process ends (destroying them at exit could outlive the runtime).
+std::map<std::string, hipGraphExec_t>& graph_cache() {
+  thread_local std::map<std::string, hipGraphExec_t>* cache =
+      new std::map<std::string, hipGraphExec_t>();
+  return *cache;
+}
+
+// Captures the launches `body` issues on `s` into a graph, runs it once (as
+// the cached executable graph of `tag`) and returns the device time of the
+// run.  Kernel timing is off inside the capture.
 template <class F>
-double run_captured(hipStream_t s, F&& body) {
+double run_captured(hipStream_t s, const std::string& tag, F&& body) {
   hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
+  const bool prof = Prof::enabled;
+  Prof::enabled = false;
   RT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
     body();
   } catch (...) {
+    Prof::enabled = prof;
     hipGraph_t tmp = nullptr;
     (void)hipStreamEndCapture(s, &tmp);
     if (tmp) (void)hipGraphDestroy(tmp);
     throw;
   }
+  Prof::enabled = prof;
   RT_HIP(hipStreamEndCapture(s, &g));
-  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    (void)hipGraphDestroy(g);
-    RT_HIP(e);
+  hipGraphExec_t& ge = graph_cache()[tag];
+  if (ge) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(ge, g, &err_node, &res) != hipSuccess ||
+        res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(ge);
+      ge = nullptr;
+    }
   }
+  if (!ge) {
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      ge = nullptr;
+      (void)hipGraphDestroy(g);
+      RT_HIP(e);
+    }
+  }
+  (void)hipGraphDestroy(g);
   hipEvent_t a, b;
   RT_HIP(hipEventCreate(&a));
   RT_HIP(hipEventCreate(&b));
@@ -2545,9 +2578,18 @@ double run_captured(hipStream_t s, F&& body) {
   RT_HIP(hipEventElapsedTime(&ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  (void)hipGraphExecDestroy(ge);
-  (void)hipGraphDestroy(g);
   return ms;
+}
+
+// Cache tag of a stage: its name, the operand shape class and the tuning
+// (which fixes the kernels launched).
+std::string stage_tag(const char* stage, bool batch, i64 m, i32 nt, const Tuning& tu) {
+  std::string t = std::string(stage) + (batch ? "/B/" : "/S/") + std::to_string(m) + "/" +
+                  std::to_string(nt);
+  for (int b = 0; b < 5; ++b)
+    t += "/" + std::to_string(tu.sym[b]) + "," + std::to_string(tu.num[b]) + "," +
+         std::to_string(tu.shape[b]);
+  return t + "/" + std::to_string(tu.learned_heavy) + "," + std::to_string(tu.learned_bins);
 }
 
 template <bool BATCH>
@@ -2573,7 +2615,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
   DBuf<i32> next(2);  // row-slot counters of the two long-row kernels
   double ms_a = 0, ms_b = 0;
   try {
-    ms_a = run_captured(s, [&] {
+    ms_a = run_captured(s, stage_tag("plain_a", BATCH, m, nt, tu), [&] {
       hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(tot.p),
                          (i64)2, (i64)0);
       launch_row_products<BATCH>(m, A.rp, A.col, B.rp, bbase, prod.p, tot.p, longq.p, s);
@@ -2600,7 +2642,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
     C.nnz = read_scalar(C.rp + m, s);
     C.col = static_cast<i32*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(i32)));
     C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
-    ms_b = run_captured(s, [&] {
+    ms_b = run_captured(s, stage_tag("plain_b", BATCH, m, nt, tu), [&] {
       bin_rows_dev(nnz.p, m, LLONG_MAX, tu.num, nb, s);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p + 1, s);
@@ -2631,7 +2673,151 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
   }
 }
 
+// The learned method in three captured stages (two host reads between them:
+// the temporary buffer's size, then C's).
+//   A: products, features, predictions, capacities, keys, scan of the
+//      capacities (temporary offsets), B's tile boundaries.
+//   B: predicted rows through the learned hash kernels (and, with
+//      learned_heavy, long rows through the learned tile kernel) into the
+//      temporary buffer; overflowed rows join the exact set; the exact rows'
+//      symbolic pass (hash bins, bitmap kernel); scan of nnz into C's row
+//      pointer.
+//   C: predicted rows copied into C; the exact rows' numeric pass.
+template <bool BATCH>
+void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
+                        const Tuning& tu, hipStream_t s, RunStats* st) {
+  const i64 m = A.rows, M = std::max<i64>(m, 1);
+  const i32 nt = (i32)std::max<i64>(1, (B.cols + TILE_W - 1) / TILE_W);
+  RT_CHECK(m < (i64)INT32_MAX, RT_E_LIMIT, "A has too many rows for 32-bit row lists");
+  RT_CHECK((double)M * nt * 4.0 < 8.0 * (1ull << 30), RT_E_LIMIT,
+           "graph mode: tile-count scratch too large; use the direct launch path");
+  std::vector<float> hp;
+  float margin = 0;
+  learned_weights(hp, margin);
+  DBuf<float> params(hp.size());
+  RT_HIP(hipMemcpyAsync(params.p, hp.data(), hp.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  DBuf<i64> prod(M), nnz(M), cap(M), key_light(M), key_heavy(M), key_exact(M), key_num(M),
+      tmp_off(m + 1);
+  DBuf<i32> ovf(M);
+  DBuf<float> feat(M * RT_NFEAT), pred(M);
+  DBuf<unsigned long long> tot(2), counts(2);
+  DBuf<i32> longq(row_products_queue(A.nnz));
+  DevBins lb(m), hb(m), sb(m), nb(m);
+  DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
+  DBuf<i64> part(scan_partials(m));
+  DBuf<i32> next(3);  // row-slot counters: learned tile, symbolic bitmap, numeric tile
+  const i64 long_th = std::min(tu.sym[4], tu.num[4]);
+  const unsigned heavy_grid = (unsigned)device().cus * 2;
+  HashArgs base = make_args(A, B, bbase);
+  C.rows = m;
+  C.cols = B.cols;
+  C.rp = static_cast<i64*>(dmalloc((m + 1) * sizeof(i64)));
+  DBuf<i32> tcol;
+  DBuf<float> tval;
+  double ms[3] = {0, 0, 0};
+  try {
+    ms[0] = run_captured(s, stage_tag("learned_a", BATCH, m, nt, tu), [&] {
+      hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(tot.p),
+                         (i64)2, (i64)0);
+      hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(256), 0, s, reinterpret_cast<i64*>(counts.p),
+                         (i64)2, (i64)0);
+      launch_row_products<BATCH>(m, A.rp, A.col, B.rp, bbase, prod.p, tot.p, longq.p, s);
+      hipLaunchKernelGGL(k_fill_i64, dim3(fill_grid(M)), dim3(256), 0, s, nnz.p, M, (i64)0);
+      learned_predict_launch(A, B, bbase, prod.p, params.p, feat.p, pred.p,
+                             tu.learned_heavy ? LLONG_MAX : long_th, s);
+      learned_capacity_launch(m, prod.p, B.cols, pred.p, margin, tu, cap.p, key_light.p,
+                              key_heavy.p, key_exact.p, ovf.p, s);
+      exclusive_scan_i64(cap.p, tmp_off.p, m, s, part.p);
+      if (B.rows > 0)
+        hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)),
+                           dim3(256), 0, s, B.rows, nt, B.rp, B.col, slab.p);
+      RT_HIP(hipGetLastError());
+    });
+    const i64 tmp_total = read_scalar(tmp_off.p + m, s);
+    tcol.alloc(std::max<i64>(tmp_total, 1));
+    tval.alloc(std::max<i64>(tmp_total, 1));
+    ms[1] = run_captured(s, stage_tag("learned_b", BATCH, m, nt, tu), [&] {
+      HashArgs a = base;
+      a.crp = tmp_off.p;
+      a.ccol = tcol.p;
+      a.cval = tval.p;
+      a.cap = cap.p;
+      a.ovf = ovf.p;
+      a.nnz = nnz.p;
+      bin_rows_dev(key_light.p, m, LLONG_MAX, tu.num, lb, s);
+      launch_hash_bins_dev<MODE_LEARN, BATCH>(a, lb, m, tu.shape, s);
+      if (tu.learned_heavy) {
+        const i64 zero[5] = {0, 0, 0, 0, 0};  // every row with a key lands in bin 5
+        bin_rows_dev(key_heavy.p, m, LLONG_MAX, zero, hb, s);
+        zero_slot(next.p, s);
+        hipLaunchKernelGGL((k_num_heavy<BATCH, true>), dim3(heavy_grid), dim3(HWG), 0, s,
+                           RowSet{hb.list.p, 0, hb.cnt.p, hb.off.p, 5}, nt, nullptr, key_heavy.p,
+                           (i64)0, A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p,
+                           tmp_off.p, tcol.p, tval.p, cap.p, nnz.p, ovf.p, next.p);
+      }
+      learned_merge_launch(m, prod.p, cap.p, ovf.p, key_exact.p, counts.p, s);
+      bin_rows_dev(key_exact.p, m, B.cols, tu.sym, sb, s);
+      HashArgs y = base;
+      y.nnz = nnz.p;
+      launch_hash_bins_dev<MODE_SYM, BATCH>(y, sb, m, tu.shape, s);
+      zero_slot(next.p + 1, s);
+      launch_sym_bitmap<BATCH>(heavy_grid, RowSet{sb.list.p, 0, sb.cnt.p, sb.off.p, 5}, A, B,
+                               bbase, slab.p, nt, tilecnt.p, nnz.p, nullptr, nullptr, next.p + 1,
+                               s);
+      exclusive_scan_i64(nnz.p, C.rp, m, s, part.p);
+      RT_HIP(hipGetLastError());
+    });
+    C.nnz = read_scalar(C.rp + m, s);
+    C.col = static_cast<i32*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(i32)));
+    C.val = static_cast<float*>(dmalloc(std::max<i64>(C.nnz, 1) * sizeof(float)));
+    ms[2] = run_captured(s, stage_tag("learned_c", BATCH, m, nt, tu), [&] {
+      learned_finish_launch(m, cap.p, ovf.p, tmp_off.p, tcol.p, tval.p, key_exact.p, nnz.p,
+                            key_num.p, C, s);
+      bin_rows_dev(key_num.p, m, LLONG_MAX, tu.num, nb, s);
+      zero_slot(next.p + 2, s);
+      hipLaunchKernelGGL((k_num_heavy<BATCH, false>), dim3(heavy_grid), dim3(HWG), 0, s,
+                         RowSet{sb.list.p, 0, sb.cnt.p, sb.off.p, 5}, nt, tilecnt.p, key_num.p,
+                         tu.num[4], A.rp, A.col, A.val, B.rp, B.col, B.val, bbase, slab.p, C.rp,
+                         C.col, C.val, nullptr, nullptr, nullptr, next.p + 2);
+      HashArgs a = base;
+      a.crp = C.rp;
+      a.ccol = C.col;
+      a.cval = C.val;
+      launch_hash_bins_dev<MODE_NUM, BATCH>(a, nb, m, tu.shape, s);
+      RT_HIP(hipGetLastError());
+    });
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    throw;
+  }
+  unsigned long long h[4];
+  RT_HIP(hipMemcpyAsync(h, tot.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipMemcpyAsync(h + 2, counts.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  RT_HIP(hipStreamSynchronize(s));
+  if (st) {
+    st->products = (i64)h[0];
+    st->max_row_products = (i64)h[1];
+    st->predicted_rows = (i64)h[2];
+    st->overflow_rows = (i64)h[3];
+  }
+  if (Prof::enabled) {
+    Prof::record("graph_learned_a", ms[0]);
+    Prof::record("graph_learned_b", ms[1]);
+    Prof::record("graph_learned_c", ms[2]);
+  }
+}
+
 }  // namespace
+
+void spgemm_learned_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                          const Tuning& tune, hipStream_t s, RunStats* st) {
+  if (tune.learned_bins == 0 && !tune.learned_heavy)  // no row predicted: the exact method
+    spgemm_plain_graph(A, B, batch, C, tune, s, st);
+  else if (batch.b_base)
+    learned_graph_impl<true>(A, B, batch.b_base, C, tune, s, st);
+  else
+    learned_graph_impl<false>(A, B, nullptr, C, tune, s, st);
+}
 
 void spgemm_plain_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                         const Tuning& tune, hipStream_t s, RunStats* st) {

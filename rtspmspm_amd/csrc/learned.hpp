@@ -27,4 +27,19 @@ std::string tuning_str(const Tuning& t);
 std::string tuning_report(const Tuning& t);
 void tune_cache_check(const std::string& path, int& valid, int& rejected);
 
+// Launch wrappers of the learned pipeline's kernels (k_learned.hip): kernels
+// only, for the graph-captured learned path.
+void learned_weights(std::vector<float>& params, float& margin);
+void learned_predict_launch(const Csr& A, const Csr& B, const i64* bbase, const i64* prod,
+                            const float* params, float* feat, float* pred, i64 skip_u,
+                            hipStream_t s);
+void learned_capacity_launch(i64 m, const i64* prod, i64 ncols, const float* pred, float margin,
+                             const Tuning& tu, i64* cap, i64* key_light, i64* key_heavy,
+                             i64* key_exact, i32* ovf, hipStream_t s);
+void learned_merge_launch(i64 m, const i64* prod, const i64* cap, const i32* ovf, i64* key_exact,
+                          unsigned long long* counts, hipStream_t s);
+void learned_finish_launch(i64 m, const i64* cap, const i32* ovf, const i64* tmp_off,
+                           const i32* tcol, const float* tval, const i64* key_exact,
+                           const i64* nnz, i64* key_num, const CsrOut& C, hipStream_t s);
+
 }  // namespace rt

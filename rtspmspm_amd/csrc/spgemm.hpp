@@ -116,6 +116,10 @@ void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
 void spgemm_plain_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                         const Tuning& tune, hipStream_t s, RunStats* st);
 
+// The learned method with its three stages captured in hipGraphs.
+void spgemm_learned_graph(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
+                          const Tuning& tune, hipStream_t s, RunStats* st);
+
 // Faithful one-work-item-per-A-nonzero launch (reference raygen + any-hit).
 void spgemm_ray(const Csr& A, const Csr& B, CsrOut& C, int ablation, int output,
                 hipStream_t s, RunStats* st);
