@@ -171,11 +171,13 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
   double best_ms = 0;
   int n = 0;
   std::ostringstream os;
-  auto trial = [&](const Tuning& t, const char* what) {
+  // A later candidate replaces the best when it is faster by more than 1%
+  // (noise); `simpler` candidates (no prediction at all) win ties.
+  auto trial = [&](const Tuning& t, const char* what, bool simpler = false) {
     const double ms = eval(t);  // < 0: the candidate's C differed (rejected)
     os << (n ? ", " : "") << what << ":" << (ms < 0 ? std::string("rejected") : std::to_string(ms));
     ++n;
-    if (ms >= 0 && (best_ms == 0 || ms < best_ms * 0.99)) {
+    if (ms >= 0 && (best_ms == 0 || ms < best_ms * (simpler ? 1.0 : 0.99))) {
       best = t;
       best_ms = ms;
     }
@@ -204,16 +206,18 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
   }
   if (learned) {
     // Learned options: long rows sized by the predictor; predictions only
-    // for bins 0-2 (short rows) or for none (every row exact).
+    // for the rows of bins 2-4 or 3-4 (rows with many products per output,
+    // where a symbolic pass costs more than the features, the prediction and
+    // the copy that replace it), or for none (every row exact).
     Tuning t = best;
     t.learned_heavy = 1;
     trial(t, "learned_heavy");
-    const int masks[2] = {7, 0};
+    const int masks[3] = {28, 24, 0};
     for (int mk : masks) {
       Tuning u = best;
       u.learned_bins = mk;
       const std::string w = "learned_bins=" + std::to_string(mk);
-      trial(u, w.c_str());
+      trial(u, w.c_str(), mk == 0);
     }
   }
   log = "best " + std::to_string(best_ms) + " ms of " + std::to_string(n) +

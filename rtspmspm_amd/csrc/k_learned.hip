@@ -56,13 +56,14 @@ __global__ __launch_bounds__(256) void k_features(i64 m, const i64* __restrict__
                                                   const i64* __restrict__ brp,
                                                   const i32* __restrict__ bcol,
                                                   const i64* __restrict__ bbase, i64 ncols,
-                                                  const i64* __restrict__ prod, i64 skip_u,
-                                                  float* __restrict__ feat) {
+                                                  const i64* __restrict__ prod, i64 skip_lo,
+                                                  i64 skip_u, float* __restrict__ feat) {
   constexpr int G = 16;
   const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / G;
   const int l = threadIdx.x & (G - 1);
   if (row >= m) return;  // whole group leaves together
-  if (min(prod[row], ncols) > skip_u) {  // a row the caller sends to the exact path unseen
+  const i64 u_row = min(prod[row], ncols);
+  if (u_row > skip_u || u_row <= skip_lo) {  // a row the caller sends to the exact path unseen
     if (l < NF) feat[row * NF + l] = 0.f;
     return;
   }
@@ -321,7 +322,9 @@ __global__ __launch_bounds__(256) void k_merge_overflow(i64 m, const i64* __rest
 }
 
 // Copy the rows the learned pass completed from the temporary buffer into C:
-// one wave per row, four coalesced 256-byte loads in flight per array.
+// CR lanes per row (a predicted row has at most a hash table's worth of
+// entries, mostly few), two coalesced loads per array in flight per lane.
+constexpr int CR = 16;
 __global__ __launch_bounds__(256) void k_copy_rows(i64 m, const i64* __restrict__ cap,
                                                    const i32* __restrict__ ovf,
                                                    const i64* __restrict__ tmp_off,
@@ -330,17 +333,17 @@ __global__ __launch_bounds__(256) void k_copy_rows(i64 m, const i64* __restrict_
                                                    const i64* __restrict__ crp,
                                                    i32* __restrict__ ccol,
                                                    float* __restrict__ cval) {
-  constexpr int U = 4;
-  const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / WAVE;
-  const int l = threadIdx.x & (WAVE - 1);
+  constexpr int U = 2;
+  const i64 row = ((i64)blockIdx.x * 256 + threadIdx.x) / CR;
+  const int l = threadIdx.x & (CR - 1);
   if (row >= m || cap[row] == 0 || ovf[row]) return;
   const i64 src = tmp_off[row], dst = crp[row], n = crp[row + 1] - dst;
-  for (i64 q0 = l; q0 < n; q0 += U * WAVE) {
+  for (i64 q0 = l; q0 < n; q0 += U * CR) {
     i32 c[U];
     float v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const i64 q = q0 + u * WAVE;
+      const i64 q = q0 + u * CR;
       if (q < n) {
         c[u] = tcol[src + q];
         v[u] = tval[src + q];
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(256) void k_copy_rows(i64 m, const i64* __restrict_
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const i64 q = q0 + u * WAVE;
+      const i64 q = q0 + u * CR;
       if (q < n) {
         ccol[dst + q] = c[u];
         cval[dst + q] = v[u];
@@ -430,19 +433,20 @@ void upload(DBuf<T>& d, const T* h, size_t n, hipStream_t s) {
   if (n) RT_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
-// Rows whose product bound min(P_i, cols) exceeds skip_u get zero features
-// (their prediction is not used).
+// Rows whose product bound min(P_i, cols) exceeds skip_u or is at most
+// skip_lo get zero features (their prediction is not used: such rows run the
+// exact path).
 void features_dev(const Csr& A, const Csr& B, const i64* bbase, const i64* prod, float* feat,
-                  hipStream_t s, i64 skip_u = LLONG_MAX) {
+                  hipStream_t s, i64 skip_u = LLONG_MAX, i64 skip_lo = -1) {
   if (A.rows == 0) return;
   const dim3 grid((unsigned)((A.rows * 16 + 255) / 256));
   ProfScope ps("row_features", s);
   if (bbase)
     hipLaunchKernelGGL(k_features<true>, grid, dim3(256), 0, s, A.rows, A.rp, A.col, B.rp, B.col,
-                       bbase, B.cols, prod, skip_u, feat);
+                       bbase, B.cols, prod, skip_lo, skip_u, feat);
   else
     hipLaunchKernelGGL(k_features<false>, grid, dim3(256), 0, s, A.rows, A.rp, A.col, B.rp, B.col,
-                       nullptr, B.cols, prod, skip_u, feat);
+                       nullptr, B.cols, prod, skip_lo, skip_u, feat);
   RT_HIP(hipGetLastError());
 }
 
@@ -463,6 +467,12 @@ void grad_dev(const float* params, const float* feat, const float* target, const
 }
 
 }  // namespace
+
+i64 predicted_floor(const Tuning& tu) {
+  int b0 = 0;
+  while (b0 < 5 && !((tu.learned_bins >> b0) & 1)) ++b0;
+  return b0 == 0 ? -1 : tu.num[b0 - 1];
+}
 
 void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
                     const Tuning& tu, hipStream_t s, RunStats* st) {
@@ -489,10 +499,20 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   RunStats& S = st ? *st : local;
   row_products(A, B, bbase, prod.p, s, &S);
   RT_HIP(hipMemsetAsync(nnz.p, 0, M * sizeof(i64), s));
+  if (!tu.learned_heavy && std::min(S.max_row_products, B.cols) <= predicted_floor(tu)) {
+    // No row's bound reaches a predicted bin: the exact path, on these
+    // row products, without the learned stages.
+    HeavyRows heavy;
+    symbolic_pass(A, B, bbase, prod.p, nnz.p, tu, s, heavy);
+    alloc_c(C, nnz.p, m, B.cols, s);
+    numeric_pass(A, B, bbase, nnz.p, heavy, C, tu, s);
+    return;
+  }
   // Long rows go to the exact path unpredicted unless the learned long-row
   // kernel is chosen (the bound k_capacity applies).
   const i64 long_th = std::min(tu.sym[4], tu.num[4]);
-  features_dev(A, B, bbase, prod.p, feat.p, s, tu.learned_heavy ? LLONG_MAX : long_th);
+  features_dev(A, B, bbase, prod.p, feat.p, s, tu.learned_heavy ? LLONG_MAX : long_th,
+               predicted_floor(tu));
   predict_dev(params.p, feat.p, m, pred.p, s);
   const dim3 g256((unsigned)((M + 255) / 256));
   {
@@ -531,7 +551,7 @@ void spgemm_learned(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
   if (m > 0) {
     {
       ProfScope ps("copy_rows", s);
-      hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * WAVE + 255) / 256)), dim3(256), 0, s, m,
+      hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * CR + 255) / 256)), dim3(256), 0, s, m,
                          cap.p, ovf.p, tmp_off.p, tcol.p, tval.p, C.rp, C.col, C.val);
       RT_HIP(hipGetLastError());
     }
@@ -554,8 +574,8 @@ void learned_weights(std::vector<float>& params, float& margin) { current_weight
 
 void learned_predict_launch(const Csr& A, const Csr& B, const i64* bbase, const i64* prod,
                             const float* params, float* feat, float* pred, i64 skip_u,
-                            hipStream_t s) {
-  features_dev(A, B, bbase, prod, feat, s, skip_u);
+                            i64 skip_lo, hipStream_t s) {
+  features_dev(A, B, bbase, prod, feat, s, skip_u, skip_lo);
   predict_dev(params, feat, A.rows, pred, s);
 }
 
@@ -583,7 +603,7 @@ void learned_finish_launch(i64 m, const i64* cap, const i32* ovf, const i64* tmp
                            const i32* tcol, const float* tval, const i64* key_exact,
                            const i64* nnz, i64* key_num, const CsrOut& C, hipStream_t s) {
   if (m <= 0) return;
-  hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * WAVE + 255) / 256)), dim3(256), 0, s, m, cap,
+  hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)((m * CR + 255) / 256)), dim3(256), 0, s, m, cap,
                      ovf, tmp_off, tcol, tval, C.rp, C.col, C.val);
   hipLaunchKernelGGL(k_numeric_keys, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m,
                      key_exact, nnz, key_num);

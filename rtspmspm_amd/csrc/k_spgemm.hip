@@ -557,11 +557,13 @@ constexpr int MODE_SYM = 0, MODE_NUM = 1, MODE_LEARN = 2;
 // also counts, gives up after a probe run of PLIM slots and flags the row
 // in *ovf instead (so that no lane can spin on a crowded table; the row's
 // distinct count is checked against its predicted capacity afterwards).
+// The flag is only written here: a row that overflowed keeps inserting its
+// remaining products (each bounded by PLIM probes) and is rerun exactly, so
+// no insert waits on a read of the flag.
 template <int MODE, int T>
 __device__ __forceinline__ void hash_insert(i32* keys, float* vals, i32 c, u32 h, i32 kk, float v,
                                             int& mycnt, i32* ovf) {
   if constexpr (MODE == MODE_LEARN) {
-    if (*ovf) return;
     constexpr int PLIM = T < 64 ? T : 64;
     bool found = false;
     for (int probe = 0; probe < PLIM; ++probe) {
@@ -643,8 +645,12 @@ __device__ __forceinline__ void group_minmax(i32& mn, i32& mx, i32* scr) {
 
 template <int G, int T, int WG, int MODE, bool BATCH>
 // A 512-thread workgroup keeps two resident per CU: four waves per SIMD
-// (the second bound), so at most 128 VGPRs.
-__global__ __launch_bounds__(WG, WG >= 512 ? 4 : 1) void k_hash(HashArgs a) {
+// (the second bound), so at most 128 VGPRs.  Groups within a wave holding
+// at most 8 table slots per lane keep seven waves per SIMD (at most 72
+// VGPRs): the learned mode's extra state would otherwise cost it a wave per
+// SIMD against the exact numeric mode (measured 10% on its 512-slot bin).
+__global__ __launch_bounds__(WG, WG >= 512 ? 4 : (G <= WAVE && T / G <= 8 ? 7 : 1))
+void k_hash(HashArgs a) {
   static_assert(G <= WAVE || G == WG, "a multi-wave group must be the whole workgroup");
   constexpr bool NUMERIC = MODE != MODE_SYM;
   constexpr int R = WG / G;
@@ -1407,8 +1413,11 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
 // table of its 32-word groups (the slot of the group's first nonzero word
 // and a mask of its nonzero words).  A product's rank is two LDS reads away
 // (group -> word), so the accumulators sit in column order.  A first tile
-// holding more than SEG_DENSE entries is one "dense" step: 16384 LDS
-// accumulators indexed by column.  Output needs no search and no sort.
+// whose compressed form alone exceeds that budget is one "dense" step: 16384
+// LDS accumulators indexed by column.  (Choosing dense steps more often --
+// for tiles of more than 1K to 8K entries -- measured slower on R-MAT s20:
+// the row then takes more, narrower steps, and per-step work dominates.)
+// Output needs no search and no sort.
 //
 // Rows of up to HC*WG entries keep their entries in registers for the whole
 // row: B row start, slab row, A value, and a cursor into the B row that
@@ -1418,7 +1427,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy(
 // loaded while the current step runs.  Longer rows re-read their entries per
 // step and chunk.
 constexpr int SEG_MAXT = 64;      // tiles per compressed step: one per lane of a planning wave
-constexpr int SEG_DENSE = 8192;   // a first tile with more entries is a dense step
+constexpr int SEG_DENSE = TILE_W;  // a first tile with more entries is a dense step
 constexpr int HSEG_LDS = TILE_W * 4;
 constexpr int SEG_BUDGET = HSEG_LDS - 8;             // compressed-step bytes (alignment slack)
 constexpr int SEG_GPT = TILE_W / 32 / 32;            // 32-word groups per tile
@@ -2724,7 +2733,7 @@ void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
       launch_row_products<BATCH>(m, A.rp, A.col, B.rp, bbase, prod.p, tot.p, longq.p, s);
       hipLaunchKernelGGL(k_fill_i64, dim3(fill_grid(M)), dim3(256), 0, s, nnz.p, M, (i64)0);
       learned_predict_launch(A, B, bbase, prod.p, params.p, feat.p, pred.p,
-                             tu.learned_heavy ? LLONG_MAX : long_th, s);
+                             tu.learned_heavy ? LLONG_MAX : long_th, predicted_floor(tu), s);
       learned_capacity_launch(m, prod.p, B.cols, pred.p, margin, tu, cap.p, key_light.p,
                               key_heavy.p, key_exact.p, ovf.p, s);
       exclusive_scan_i64(cap.p, tmp_off.p, m, s, part.p);

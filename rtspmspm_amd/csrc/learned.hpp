@@ -32,7 +32,11 @@ void tune_cache_check(const std::string& path, int& valid, int& rejected);
 void learned_weights(std::vector<float>& params, float& margin);
 void learned_predict_launch(const Csr& A, const Csr& B, const i64* bbase, const i64* prod,
                             const float* params, float* feat, float* pred, i64 skip_u,
-                            hipStream_t s);
+                            i64 skip_lo, hipStream_t s);
+// Rows whose bound min(P, cols) is at most this never land in a predicted
+// bin (their capacity is below the lowest one learned_bins selects): the
+// learned method skips their features (-1: none skipped).
+i64 predicted_floor(const Tuning& tu);
 void learned_capacity_launch(i64 m, const i64* prod, i64 ncols, const float* pred, float margin,
                              const Tuning& tu, i64* cap, i64* key_light, i64* key_heavy,
                              i64* key_exact, i32* ovf, hipStream_t s);

@@ -115,3 +115,54 @@ def test_trained_network_tightens_tables(trained):
           f"({cap.sum() / U[sel].sum():.2f}x), exact {nnz[sel].sum()}, overflow {over:.3f}")
     assert cap.sum() < U[sel].sum()
     assert over < 0.05
+
+
+def test_trained_network_gains_time_where_rows_have_many_products():
+    """What the trained network gains, in time.  C = A*A for a 27-point
+    stencil: 729 products and 125 entries per interior row.  A network
+    trained in-test on the 24^3 stencil square predicts the rows' sizes, so
+    the learned method sizes them into 512-slot tables, where the upper bound
+    min(P, cols) = 729 needs 2048-slot ones; it then also beats the exact
+    method, whose symbolic pass over the 729 products it replaces with
+    features, a prediction and a copy of C.  C equals the exact method's."""
+    shipped = rt.mlp_get_weights()
+    try:
+        T = rt.gen_stencil27(24, 5)
+        F, nnz = rt.row_features(T, T)
+        keep = nnz > 0
+        rt.mlp_train(F[keep], np.log2(1.0 + nnz[keep]).astype(np.float32), epochs=60,
+                     batch=1024, lr=3e-3, seed=0)
+        trained = rt.mlp_get_weights()
+        S = rt.gen_stencil27(48, 6)
+        Cl, info = rt.spmspm(S, S, mode="learned")
+        assert info["predicted_rows"] == S.shape[0] and info["overflow_rows"] == 0
+        check_parity(Cl, oracle.spgemm_f64(S, S))
+
+        A = rt.gen_stencil27(96, 7)
+        dA = rt.to_device(A)
+
+        def median_ms(mode):
+            for _ in range(2):
+                C, _ = rt.spmspm(dA, dA, mode=mode, trusted=True)
+                C.free()
+            ms = []
+            for _ in range(7):
+                C, i = rt.spmspm(dA, dA, mode=mode, trusted=True)
+                ms.append(i["ms"])
+                C.free()
+            return float(np.median(ms))
+        plain = median_ms("plain")
+        learned = median_ms("learned")
+        rt.mlp_set_weights(np.zeros_like(trained[0]), 0.0)  # upper-bound sizing
+        bound = median_ms("learned")
+        rt.mlp_set_weights(*trained)
+        Cp = rt.spmspm(dA, dA, mode="plain")[0].row_ptr_host()
+        Cq = rt.spmspm(dA, dA, mode="learned")[0].row_ptr_host()
+        assert np.array_equal(Cp, Cq)
+        print(f"stencil27 96^3 A*A: exact {plain:.3f} ms, learned {learned:.3f} ms "
+              f"({plain / learned:.2f}x), learned with upper-bound sizing {bound:.3f} ms "
+              f"(network gain {bound / learned:.2f}x)")
+        assert learned < 0.8 * bound
+        assert learned < 1.1 * plain
+    finally:
+        rt.mlp_set_weights(*shipped)
