@@ -612,11 +612,16 @@ struct HashArgs {
   const i64* cap;
   i32* ovf;
   // When dcount is set the kernel is persistent: its rows are the bin `bin`
-  // of a device-side binning (count dcount[bin], first row list[doff[bin]]).
+  // of a device-side binning (count dcount[bin], first row list[doff[bin]]),
+  // and its workgroups take runs of TICKET_RUN row groups in turn from the
+  // counter ticket[bin] (zeroed with the binning), so that one that drew
+  // cheap rows takes more.
   const i32* dcount;
   const i32* doff;
   int bin;
+  i32* ticket;
 };
+constexpr int TICKET_RUN = 32;
 
 // Minimum and maximum over a group of G lanes (G <= 64), or over the whole
 // workgroup (G == WG, through scr[WG/64]); every lane receives both.
@@ -701,9 +706,31 @@ void k_hash(HashArgs a) {
     __syncthreads();
   }
   // Row groups are taken grid-stride (a single pass when the grid covers the
-  // bin).  The loop is uniform over the workgroup; in block mode the tables
-  // are shared by the whole workgroup, hence the barrier between groups.
-  for (i64 gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+  // bin), or, for a persistent launch, in runs from the bin's ticket counter
+  // (one atomic per run: one per row group serialises on the counter).  The
+  // loop is uniform over the workgroup; in block mode the tables are shared
+  // by the whole workgroup, hence the barrier between groups.
+  __shared__ i32 s_ticket;
+  i64 run_end = 0;
+  auto first = [&]() -> i64 {
+    if (!a.ticket) return (i64)blockIdx.x;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&a.ticket[a.bin], TICKET_RUN);
+    __syncthreads();
+    const i64 g0 = s_ticket;
+    run_end = g0 + TICKET_RUN;
+    return g0;
+  };
+  auto next = [&](i64 gi) -> i64 {
+    if (!a.ticket) return gi + gridDim.x;
+    if (gi + 1 < run_end) return gi + 1;
+    __syncthreads();  // every thread has read the previous ticket
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&a.ticket[a.bin], TICKET_RUN);
+    __syncthreads();
+    const i64 g0 = s_ticket;
+    run_end = g0 + TICKET_RUN;
+    return g0;
+  };
+  for (i64 gi = first(); gi < ngroups; gi = next(gi)) {
   [&]() {
   const i64 ridx = gi * R + g;
   const bool active = ridx < nlist;
@@ -2146,6 +2173,7 @@ void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr&
 
 HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
   HashArgs a{};
+  a.ticket = nullptr;
   a.arp = A.rp;
   a.acol = A.col;
   a.aval = A.val;
@@ -2461,14 +2489,16 @@ unsigned fill_grid(i64 n) {
 }
 
 struct DevBins {
-  DBuf<i32> list, cnt, off, cursor;
-  explicit DevBins(i64 m) : list(std::max<i64>(m, 1)), cnt(NBIN), off(NBIN + 1), cursor(NBIN) {}
+  DBuf<i32> list, cnt, off, cursor, ticket;
+  explicit DevBins(i64 m)
+      : list(std::max<i64>(m, 1)), cnt(NBIN), off(NBIN + 1), cursor(NBIN), ticket(NBIN) {}
 };
 
 void bin_rows_dev(const i64* x, i64 n, i64 cap, const i64 th_in[5], DevBins& b, hipStream_t s) {
   Thr th;
   for (int i = 0; i < 5; ++i) th.t[i] = th_in[i];
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.cnt.p, (i64)NBIN, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.ticket.p, (i64)NBIN, 0);
   if (n > 0) {
     const dim3 grid((unsigned)((n + 255) / 256));
     hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, b.cnt.p);
@@ -2488,6 +2518,7 @@ void launch_hash_dev(HashArgs a, const DevBins& b, int bin, i64 max_rows, hipStr
   a.dcount = b.cnt.p;
   a.doff = b.off.p;
   a.bin = bin;
+  a.ticket = b.ticket.p;
   const i64 groups = std::max<i64>(1, (max_rows + R - 1) / R);
   // Exactly the workgroups the device holds at once, so the grid-stride
   // loop runs in a single round on every CU.
