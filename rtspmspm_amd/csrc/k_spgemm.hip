@@ -1048,7 +1048,11 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
   static_assert(WW / 32 == WG, "one 32-word group of the window per thread");
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
-  __shared__ u32 s_bits[WW];
+  // The window's bitmap, with one spare word after every 32-word group: word
+  // w sits at w + w / 32, so group g starts on bank g mod 32 and the
+  // threads reading their own groups in the epilogue do not collide.
+  constexpr int GW = 33;  // stored words per group
+  __shared__ u32 s_bits[WW / 32 * GW];
   __shared__ OffT s_off[WG];
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
@@ -1071,7 +1075,7 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
       const i64 win = (i64)tw0 * TILE_W;
       const i64 wcols = min((i64)SYM_WIN, ncols - win);
       const int nd = (int)((wcols + 31) / 32);
-      for (int d = l; d < nd; d += WG) s_bits[d] = 0;
+      for (int d = l; d < (nd + 31) / 32 * GW; d += WG) s_bits[d] = 0;
       __syncthreads();
       for (i64 e0 = a0; e0 < a1; e0 += WG) {
         const i64 e = e0 + l;
@@ -1091,7 +1095,7 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
           for (int r = 0; r < ROWS; ++r) {
             const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
             const u32 bit = ok[r] ? 1u << (x & 31) : 0u;
-            if (bit) lds_or_u32(&s_bits[x >> 5], bit);
+            if (bit) lds_or_u32(&s_bits[(x >> 5) + (x >> 10)], bit);
           }
         });
       }
@@ -1104,15 +1108,9 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
       int gent = 0;
       {
         const int lim = nd - l * 32;
-        const uint4* q = reinterpret_cast<const uint4*>(s_bits) + l * 8;
+        const u32* q = s_bits + l * GW;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const uint4 x = q[j];
-          gw[4 * j] = x.x;
-          gw[4 * j + 1] = x.y;
-          gw[4 * j + 2] = x.z;
-          gw[4 * j + 3] = x.w;
-        }
+        for (int j = 0; j < 32; ++j) gw[j] = q[j];
 #pragma unroll
         for (int j = 0; j < 32; ++j) {
           if (j >= lim) gw[j] = 0u;

@@ -224,6 +224,45 @@ def test_long_row_order_with_hub_rows(mode):
     check_parity(C, oracle.spgemm_f64(A, B))
 
 
+def _span_operands(seed):
+    """B rows of N entries, N from 5 to 4,000 (rows of every hash bin), whose
+    columns lie in a span of about 2N columns, 64N, 256N or the whole 2^22:
+    the hash kernels order such rows with one bitmap window, with several,
+    or with the sort.  Families of four B rows share a span; A rows take one
+    of them or two (overlapping columns: merged products)."""
+    rng = np.random.default_rng(seed)
+    ncols = 1 << 22
+    br, bc, ar, ac = [], [], [], []
+    nb = na = 0
+    for n in [5, 12, 40, 100, 200, 600, 1000, 2000, 4000]:
+        for span in [2 * n + 8, 64 * n, 256 * n, ncols]:
+            span = min(span, ncols)
+            base = int(rng.integers(0, ncols - span + 1))
+            for k in range(4):
+                cols = base + np.sort(rng.choice(span, n, replace=False))
+                br.append(np.full(n, nb + k))
+                bc.append(cols)
+            for pick in ([0], [1], [0, 1], [2, 3]):
+                ar.append(np.full(len(pick), na))
+                ac.append(nb + np.array(pick))
+                na += 1
+            nb += 4
+    br, bc = np.concatenate(br), np.concatenate(bc)
+    B = csr_from_coo(nb, ncols, br, bc, rng.uniform(-1, 1, br.size))
+    ar, ac = np.concatenate(ar), np.concatenate(ac)
+    A = csr_from_coo(na, nb, ar, ac, rng.uniform(-1, 1, ar.size))
+    return A, B
+
+
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_hash_rows_over_narrow_windowed_and_wide_spans(mode):
+    A, B = _span_operands(31)
+    C, _ = run(A, B, mode=mode)
+    check_parity(C, oracle.spgemm_f64(A, B))
+    Cg, _ = run(A, B, mode="plain", graph=True)
+    assert np.array_equal(Cg.row_ptr, C.row_ptr) and np.array_equal(Cg.col, C.col)
+
+
 def test_structure_is_deterministic():
     A = rt.gen_rmat(13, 16, seed=8)
     C1, _ = run(A, A)
