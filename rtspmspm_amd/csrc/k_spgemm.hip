@@ -1089,13 +1089,24 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
           len = sk[tw1] - s0;
         }
         const int nvalid = (int)min((i64)WG, a1 - e0);
+        // Lanes 2k and 2k+1 hold neighbouring products (consecutive entries
+        // of a B row, often in one word when the row is dense): when their
+        // words match, the even lane sets both bits and the odd lane none,
+        // which halves the same-address atomics a run of one word costs.
+        // (Merging quads as well, through a second exchange, measured no
+        // faster: the added VALU work cost what the atomics saved.)
         chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl,
                                         [&](const auto& c, const auto&, const auto& ok) {
+          const bool odd = threadIdx.x & 1;
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) {
             const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
-            const u32 bit = ok[r] ? 1u << (x & 31) : 0u;
-            if (bit) lds_or_u32(&s_bits[(x >> 5) + (x >> 10)], bit);
+            const int w = x >> 5;
+            u32 bit = ok[r] ? 1u << (x & 31) : 0u;
+            const int wp = __builtin_amdgcn_update_dpp(0, w, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+            const u32 bp = (u32)__builtin_amdgcn_update_dpp(0, (int)bit, 0xB1, 0xf, 0xf, false);
+            if (wp == w) bit = odd ? 0u : bit | bp;
+            if (bit) lds_or_u32(&s_bits[w + (w >> 5)], bit);
           }
         });
       }
