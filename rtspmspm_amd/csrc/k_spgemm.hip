@@ -1607,11 +1607,25 @@ template <int WG>
 __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n, i64 out,
                                             i64 cbase, float* acc, i32* __restrict__ ccol,
                                             float* __restrict__ cval) {
+  constexpr int U = 4;  // LDS reads issued before their stores
   const int l = threadIdx.x;
   const int q0 = l - (int)(out & 31);
-  for (int q = q0; q < n; q += WG)
-    if (q >= 0) __builtin_nontemporal_store(acc[q], &cval[out + q]);
+  for (int qb = q0; qb < n; qb += U * WG) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = qb + u * WG;
+      v[u] = q >= 0 && q < n ? acc[q] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = qb + u * WG;
+      if (q >= 0 && q < n) __builtin_nontemporal_store(v[u], &cval[out + q]);
+    }
+  }
   __syncthreads();
+  // Column offsets of the staged words' entries, two set bits per round
+  // (a wave runs as many rounds as its densest word needs).
   i32* colx = reinterpret_cast<i32*>(acc);
   for (int slot = l; slot < nwords; slot += WG) {
     const u64 e = s_cwd[slot];
@@ -1619,15 +1633,31 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
     int q = (int)(e >> 47);
     const int x0 = (int)((e >> 32) & 0x7fff) * 32;
     while (bits) {
-      colx[q++] = x0 + (__ffs(bits) - 1);
+      const int b0 = __ffs(bits) - 1;
       bits &= bits - 1;
+      colx[q] = x0 + b0;
+      if (bits) {
+        colx[q + 1] = x0 + (__ffs(bits) - 1);
+        bits &= bits - 1;
+      }
+      q += 2;
     }
   }
   __syncthreads();
-  for (int q = q0; q < n; q += WG) {
-    if (q >= 0) {
-      __builtin_nontemporal_store((i32)(cbase + colx[q]), &ccol[out + q]);
-      colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
+  for (int qb = q0; qb < n; qb += U * WG) {
+    i32 c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = qb + u * WG;
+      c[u] = q >= 0 && q < n ? colx[q] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = qb + u * WG;
+      if (q >= 0 && q < n) {
+        __builtin_nontemporal_store((i32)(cbase + c[u]), &ccol[out + q]);
+        colx[q] = 0;  // also 0.0f: the accumulators start the next step at zero
+      }
     }
   }
   __syncthreads();
