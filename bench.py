@@ -174,6 +174,35 @@ def _timed_product(call, steps, warmup):
     return (time.perf_counter() - t0) / steps * 1e3, float(np.median(dev)), info, nnz
 
 
+def _timed_methods(calls, steps, warmup):
+    """Times several methods of one product fairly: warm-ups, then `steps`
+    rounds that call each method once in turn (so drift in clocks or
+    temperature falls on all alike).  calls: name -> call() -> (C, info,
+    free).  Returns name -> (median wall ms per call, median device ms,
+    last info, last C's nnz)."""
+    import rtspmspm_amd as rt
+    for call in calls.values():
+        for _ in range(warmup):
+            C, info, free = call()
+            free()
+    rt.synchronize()
+    wall = {k: [] for k in calls}
+    dev = {k: [] for k in calls}
+    last = {}
+    for _ in range(steps):
+        for k, call in calls.items():
+            t0 = time.perf_counter()
+            C, info, free = call()
+            nnz = C.nnz
+            free()
+            rt.synchronize()
+            wall[k].append((time.perf_counter() - t0) * 1e3)
+            dev[k].append(info["ms"])
+            last[k] = (info, nnz)
+    return {k: (float(np.median(wall[k])), float(np.median(dev[k])), last[k][0], last[k][1])
+            for k in calls}
+
+
 def _entry(ms, dev_ms, flops, bytes_alg, info=None):
     e = {"ms": round(ms, 4), "device_ms": round(dev_ms, 4), "gflops": round(flops / ms / 1e6, 3),
          "roofline_frac": round(bytes_alg / ms / 1e6 / HBM_PEAK_GBS, 6)}
@@ -202,6 +231,7 @@ def wl_single(name, A, B, desc, steps, warmup, tune, modes=("plain", "learned"))
     bytes_alg = (W.csr_bytes(A.shape[0], A.nnz) + (0 if B is A else W.csr_bytes(B.shape[0], B.nnz))
                  + W.csr_bytes(A.shape[0], nnz))
     out = {"config": desc, "products": P, "nnz_C": nnz, "bytes_alg": bytes_alg, "tuning": {}}
+    calls = {}
     for mode in modes:
         if tune and mode != "ray":
             out["tuning"][mode] = rt.autotune(dA, dB, mode=mode, reps=1).split("; ", 1)[-1]
@@ -209,7 +239,8 @@ def wl_single(name, A, B, desc, steps, warmup, tune, modes=("plain", "learned"))
         def call(mode=mode):
             C, info = rt.spmspm(dA, dB, mode=mode, use_tuned=tune, trusted=True)
             return C, info, C.free
-        ms, dev_ms, info, got = _timed_product(call, steps, warmup)
+        calls[mode] = call
+    for mode, (ms, dev_ms, info, got) in _timed_methods(calls, steps, warmup).items():
         assert got == nnz, f"{name} {mode}: nnz(C) {got} != {nnz}"
         out[mode] = _entry(ms, dev_ms, 2 * P, bytes_alg, info)
         log(f"  {name} {mode}: {ms:.3f} ms ({dev_ms:.3f} device), {2 * P / ms / 1e6:.1f} GFLOP/s")
@@ -233,6 +264,7 @@ def wl_galerkin(grid, steps, warmup, tune):
     out["bytes_alg"] = (W.csr_bytes(A.shape[0], A.nnz) + W.csr_bytes(Pm.shape[0], Pm.nnz)
                         + 2 * W.csr_bytes(A.shape[0], nnz_ap) + W.csr_bytes(R.shape[0], R.nnz)
                         + W.csr_bytes(R.shape[0], nnz_c))
+    calls = {}
     for mode in ("plain", "learned"):
         if tune:
             out["tuning"][mode] = [rt.autotune(dA, dP, mode=mode, reps=1).split("; ", 1)[-1],
@@ -246,7 +278,8 @@ def wl_galerkin(grid, steps, warmup, tune):
                     "predicted_rows": i1["predicted_rows"] + i2["predicted_rows"],
                     "overflow_rows": i1["overflow_rows"] + i2["overflow_rows"]}
             return C, info, C.free
-        ms, dev_ms, info, got = _timed_product(call, steps, warmup)
+        calls[mode] = call
+    for mode, (ms, dev_ms, info, got) in _timed_methods(calls, steps, warmup).items():
         assert got == nnz_c
         out[mode] = _entry(ms, dev_ms, 2 * (P1 + P2), out["bytes_alg"], info)
         log(f"  galerkin {mode}: {ms:.3f} ms ({dev_ms:.3f} device)")
@@ -275,6 +308,7 @@ def wl_batch(count, steps, warmup, tune):
                  + W.csr_bytes(SA.shape[0], nnz_c))
     out = {"config": f"{count} pairs, n log-uniform [256, 65536], Poisson(U(4,32)) per row, "
                      "seed 4", "products": P, "nnz_C": nnz_c, "bytes_alg": bytes_alg, "tuning": {}}
+    calls = {}
     for mode in ("plain", "learned"):
         if tune:
             out["tuning"]["batched_" + mode] = rt.autotune_batched(
@@ -284,7 +318,8 @@ def wl_batch(count, steps, warmup, tune):
             C, info = rt.spmspm_batched(ar, br, bc, dSA, dSB, mode=mode, use_tuned=tune,
                                         trusted=True)
             return C, info, C.free
-        ms, dev_ms, info, got = _timed_product(call, steps, warmup)
+        calls[mode] = call
+    for mode, (ms, dev_ms, info, got) in _timed_methods(calls, steps, warmup).items():
         assert got == nnz_c
         out["batched_" + mode] = _entry(ms, dev_ms, 2 * P, bytes_alg, info)
         log(f"  batched {mode}: {ms:.2f} ms ({dev_ms:.2f} device), {2 * P / ms / 1e6:.1f} GFLOP/s")
@@ -313,20 +348,23 @@ def wl_batch(count, steps, warmup, tune):
 def run_workloads(names, args, tune):
     from rtspmspm_amd import workloads as W
     res = {}
+    # Products of a few ms or less take more timed calls (a few microseconds
+    # of host noise would otherwise decide their method ratios).
+    small = max(args.steps, 20)
     for name in names:
         t0 = time.perf_counter()
         if name == "uniform4k":
             A = W.uniform4k()
             res[name] = wl_single(name, A, A, "A*A, A 4096 x 4096, 8 uniform columns per row, "
-                                  "U(-1,1), seed 0", args.steps, args.warmup, tune,
+                                  "U(-1,1), seed 0", max(args.steps, 50), args.warmup, tune,
                                   ("plain", "learned", "ray"))
         elif name == "galerkin":
-            res[name] = wl_galerkin(args.grid, args.steps, args.warmup, tune)
+            res[name] = wl_galerkin(args.grid, small, args.warmup, tune)
         elif name == "rect":
             A, B = W.rect()
             res[name] = wl_single(name, A, B, "A 500000 x 2000000 lognormal(2.5,1) rows * "
                                   "B 2000000 x 500000 4 per row, N(0,1), seed 3",
-                                  args.steps, args.warmup, tune)
+                                  small, args.warmup, tune)
         elif name == "batch2048":
             res[name] = wl_batch(args.count, args.steps, args.warmup, tune)
         res[name]["seconds"] = round(time.perf_counter() - t0, 1)

@@ -172,30 +172,38 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
   int n = 0;
   std::ostringstream os;
   // A later candidate replaces the best when it is faster by more than 1%
-  // (noise); `simpler` candidates (no prediction at all) win ties.
-  auto trial = [&](const Tuning& t, const char* what, bool simpler = false) {
+  // (noise).
+  auto trial = [&](const Tuning& t, const char* what) {
     const double ms = eval(t);  // < 0: the candidate's C differed (rejected)
     os << (n ? ", " : "") << what << ":" << (ms < 0 ? std::string("rejected") : std::to_string(ms));
     ++n;
-    if (ms >= 0 && (best_ms == 0 || ms < best_ms * (simpler ? 1.0 : 0.99))) {
+    if (ms >= 0 && (best_ms == 0 || ms < best_ms * 0.99)) {
       best = t;
       best_ms = ms;
     }
   };
   // Thresholds: default; long-row cut at 2048 / 1024; load factor 1/4 in
   // bins 0-3 (thresholds at a quarter of the tables), alone and with the
-  // 2048 cut.
+  // 2048 cut.  For the learned method they are searched with every row
+  // exact (learned_bins 0, the exact shortcut), so its search starts from
+  // the plain method's own best.
   const i64 sets[5][5] = {{16, 64, 256, 1024, 4096},
                           {16, 64, 256, 1024, 2048},
                           {16, 64, 256, 512, 1024},
                           {8, 32, 128, 512, 4096},
                           {8, 32, 128, 512, 2048}};
   const char* names[5] = {"default", "cut2048", "cut1024", "load1/4", "load1/4+cut2048"};
-  for (int c = 0; c < 5; ++c) {
-    Tuning t;
-    for (int b = 0; b < 5; ++b) t.sym[b] = t.num[b] = sets[c][b];
-    trial(t, names[c]);
-  }
+  auto threshold_sets = [&](const Tuning& base, const char* tag) {
+    for (int c = 0; c < 5; ++c) {
+      Tuning t = base;
+      for (int b = 0; b < 5; ++b) t.sym[b] = t.num[b] = sets[c][b];
+      const std::string w = std::string(tag) + names[c];
+      trial(t, w.c_str());
+    }
+  };
+  Tuning start;
+  if (learned) start.learned_bins = 0;
+  threshold_sets(start, "");
   RT_CHECK(best_ms > 0, RT_E_INVALID, "autotune: no candidate produced a valid result");
   // Workgroup shape, one bin at a time.
   for (int b = 0; b < 5; ++b) {
@@ -205,20 +213,23 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
     trial(t, w.c_str());
   }
   if (learned) {
-    // Learned options: long rows sized by the predictor; predictions only
-    // for the rows of bins 2-4 or 3-4 (rows with many products per output,
-    // where a symbolic pass costs more than the features, the prediction and
-    // the copy that replace it), or for none (every row exact).
+    // Learned options on top: predictions for the rows of every bin, of
+    // bins 2-4 or of bins 3-4 (rows with many products per output, where a
+    // symbolic pass costs more than the features, the prediction and the
+    // copy that replace it); long rows sized by the predictor.  A predicted
+    // mask that wins has the threshold sets searched again under it.
+    const Tuning exact = best;
+    const int masks[3] = {31, 28, 24};
+    for (int mk : masks) {
+      Tuning u = exact;
+      u.learned_bins = mk;
+      const std::string w = "learned_bins=" + std::to_string(mk);
+      trial(u, w.c_str());
+    }
+    if (best.learned_bins != 0) threshold_sets(best, "masked ");
     Tuning t = best;
     t.learned_heavy = 1;
     trial(t, "learned_heavy");
-    const int masks[3] = {28, 24, 0};
-    for (int mk : masks) {
-      Tuning u = best;
-      u.learned_bins = mk;
-      const std::string w = "learned_bins=" + std::to_string(mk);
-      trial(u, w.c_str(), mk == 0);
-    }
   }
   log = "best " + std::to_string(best_ms) + " ms of " + std::to_string(n) +
         " candidates; ms per candidate: " + os.str();
