@@ -74,7 +74,9 @@ struct SplitEntries {
     c = col[i];
     v = val[i];
   }
-  // Entry base + j, with base uniform (a scalar base address, a lane offset).
+  // Entry base + j, with base uniform (a scalar base address) and j the
+  // lane's offset in units of LANE_UNIT entries (j = lane * LANE_UNIT).
+  static constexpr int LANE_UNIT = 1;
   template <class OffT>
   __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float& v) const {
     c = (col + base)[j];
@@ -89,6 +91,7 @@ struct PairedEntries {
     c = (i32)x.x;
     v = __uint_as_float(x.y);
   }
+  static constexpr int LANE_UNIT = 1;
   template <class OffT>
   __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float& v) const {
     const uint2 x = (cv + base)[j];
@@ -104,9 +107,12 @@ struct ColumnEntries {
   __device__ __forceinline__ void load(OffT i, i32& c, float&) const {
     c = col[i];
   }
+  // The lane's offset comes in bytes (LANE_UNIT 4): selecting it between 0
+  // and a precomputed lane * 4 saves a shift per gather.
+  static constexpr int LANE_UNIT = 4;
   template <class OffT>
-  __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float&) const {
-    c = (col + base)[j];
+  __device__ __forceinline__ void load_at(OffT base, int jb, i32& c, float&) const {
+    c = *reinterpret_cast<const i32*>(reinterpret_cast<const char*>(col + base) + jb);
   }
 };
 
@@ -383,6 +389,7 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
     __syncthreads();
     const int per = (tb + NW - 1) / NW;
     const int pb = w * per, pe = min(tb, pb + per);
+    const int lane_b = lane * Src::LANE_UNIT;  // the lane's offset in a block, as src takes it
     int lo = 0;  // owner of the batch's first block, or an entry before it
     for (int b0 = pb; b0 < pe; b0 += WAVE) {
       const int nblk = min(WAVE, pe - b0);
@@ -412,7 +419,7 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           ok[r] = lane < __builtin_amdgcn_readlane(rem, j0 + r);
-          src.load_at(lane_value(bs, j0 + r), ok[r] ? lane : 0, c[r], v[r]);
+          src.load_at(lane_value(bs, j0 + r), ok[r] ? lane_b : 0, c[r], v[r]);
         }
         // All ROWS gathers leave before any result is used.  Without this
         // fence the scheduler moves the first product's use up between the
@@ -1103,8 +1110,8 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
             const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
             const int w = x >> 5;
             u32 bit = ok[r] ? 1u << (x & 31) : 0u;
-            const int wp = __builtin_amdgcn_update_dpp(0, w, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
-            const u32 bp = (u32)__builtin_amdgcn_update_dpp(0, (int)bit, 0xB1, 0xf, 0xf, false);
+            const int wp = __builtin_amdgcn_mov_dpp(w, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+            const u32 bp = (u32)__builtin_amdgcn_mov_dpp((int)bit, 0xB1, 0xf, 0xf, false);
             if (wp == w) bit = odd ? 0u : bit | bp;
             if (bit) lds_or_u32(&s_bits[w + (w >> 5)], bit);
           }
