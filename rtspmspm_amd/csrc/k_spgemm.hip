@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -66,7 +67,9 @@ __global__ __launch_bounds__(256) void k_pair_entries(i64 n, const i32* __restri
 
 // Where a numeric kernel gathers B's entries from: separate column and value
 // arrays, or (column, value) pairs.
+// PER is the count of consecutive entries a lane gathers per block load.
 struct SplitEntries {
+  static constexpr int PER = 1;
   const i32* __restrict__ col;
   const float* __restrict__ val;
   template <class OffT>
@@ -84,6 +87,7 @@ struct SplitEntries {
   }
 };
 struct PairedEntries {
+  static constexpr int PER = 1;
   const uint2* __restrict__ cv;
   template <class OffT>
   __device__ __forceinline__ void load(OffT i, i32& c, float& v) const {
@@ -91,10 +95,11 @@ struct PairedEntries {
     c = (i32)x.x;
     v = __uint_as_float(x.y);
   }
-  static constexpr int LANE_UNIT = 1;
+  // The lane's offset comes in bytes (LANE_UNIT 8), as for ColumnEntries.
+  static constexpr int LANE_UNIT = 8;
   template <class OffT>
-  __device__ __forceinline__ void load_at(OffT base, int j, i32& c, float& v) const {
-    const uint2 x = (cv + base)[j];
+  __device__ __forceinline__ void load_at(OffT base, int jb, i32& c, float& v) const {
+    const uint2 x = *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(cv + base) + jb);
     c = (i32)x.x;
     v = __uint_as_float(x.y);
   }
@@ -102,6 +107,7 @@ struct PairedEntries {
 
 // Column indices only (symbolic kernels): the value is left unset.
 struct ColumnEntries {
+  static constexpr int PER = 1;
   const i32* __restrict__ col;
   template <class OffT>
   __device__ __forceinline__ void load(OffT i, i32& c, float&) const {
@@ -115,6 +121,38 @@ struct ColumnEntries {
     c = *reinterpret_cast<const i32*>(reinterpret_cast<const char*>(col + base) + jb);
   }
 };
+
+// Column indices, two consecutive entries per lane in the block walk: one
+// 8-byte load at an even entry index.  The array must hold one readable
+// entry past the last one a run can end on (a pair whose second half lies
+// past a run's end is loaded whole and that half dropped).
+struct ColumnPairs {
+  static constexpr int PER = 2;
+  const i32* __restrict__ col;  // 8-byte aligned
+  template <class OffT>
+  __device__ __forceinline__ void load(OffT i, i32& c, float&) const {
+    c = col[i];
+  }
+  // Entries base, base + 1 (base even and uniform) + the lane's byte offset
+  // jb (lane * 8).
+  static constexpr int LANE_UNIT = 8;
+  template <class OffT>
+  __device__ __forceinline__ void load2_at(OffT base, int jb, i32& c0, i32& c1, float&,
+                                           float&) const {
+    const uint2 x =
+        *reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(col + base) + jb);
+    c0 = (i32)x.x;
+    c1 = (i32)x.y;
+  }
+};
+
+// B's column indices copied with one spare entry past the end (the
+// ColumnPairs source reads a pair whole).
+__global__ __launch_bounds__(256) void k_copy_cols(i64 n, const i32* __restrict__ col,
+                                                   i32* __restrict__ out) {
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i <= n; i += (i64)gridDim.x * 256)
+    out[i] = i < n ? col[i] : 0;
+}
 
 // Zeroes the row-slot counter of a one-row-per-workgroup kernel.
 void zero_slot(i32* next, hipStream_t s) {
@@ -301,7 +339,12 @@ __device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo,
 //   blocks and computes the block's first B index, length and A value; the
 //   wave then walks the batch ROWS blocks at a time, lane l taking product
 //   l of each block, with the block's fields read from lane j (v_readlane):
-//   no per-product entry lookup, and the B loads use a scalar base.
+//   no per-product entry lookup, and the B loads use a scalar base.  A
+//   source with PER = 2 (ColumnPairs) has blocks of 128 products starting
+//   at even B indices, and lane l loads products 2l and 2l + 1 of a block
+//   with one 8-byte load (half the loads and block bookkeeping per
+//   product); an entry starting at an odd index masks its first slot, and
+//   such blocks are taken when they keep at least half the lanes busy.
 // * Owner marks, otherwise (short runs).  Wave w takes an equal contiguous
 //   slice of the products and walks it in sub-batches of 64*ROWS (lane l:
 //   products 64r + l).  For a sub-batch the wave marks, in its own zeroed
@@ -362,10 +405,18 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
                                                "ends fit the mark array");
   constexpr int NW = WG / WAVE;
   constexpr bool PACK = ChunkLds<OffT>::packs(VAL);
+  // Entries per lane and block gather (a block holds BW products).  With
+  // PER 2 a block starts at an even B index, so an entry that starts at an
+  // odd one leads with one masked product (`lead`).
+  constexpr int PER = Src::PER;
+  constexpr int BW = WAVE * PER;
+  static_assert(PER == 1 || PER == 2, "one or two entries per lane");
+  static_assert(ROWS % PER == 0, "whole blocks per batch");
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1);
   const int w = __builtin_amdgcn_readfirstlane(l / WAVE);
-  const int nb = (len + WAVE - 1) / WAVE;
+  const int lead = PER == 2 && len > 0 ? (int)(start & 1) : 0;
+  const int nb = (len + lead + BW - 1) / BW;
   u64 both;
   const u64 inc = block_scan_incl<WG>(((u64)(u32)len << 32) | (u32)nb, s.scr, &both);
   const int tot = (int)(both >> 32), tb = (int)(u32)both;
@@ -373,13 +424,17 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
     __syncthreads();  // every wave has read s.scr before the next scan writes it
     return 0;
   }
-  const bool blocks = (i64)tot * 4 >= (i64)tb * (3 * WAVE);
+  // Blocks when their lanes are at least 3/4 busy (1/2 with pairs, whose
+  // block walk costs about half as many instructions per slot).
+  const bool blocks = PER == 1 ? (i64)tot * 4 >= (i64)tb * (3 * BW) : (i64)tot * 2 >= (i64)tb * BW;
   if (blocks) {
     const int bex = (int)(u32)inc - nb;
     i32* s_end = reinterpret_cast<i32*>(s.mark);
     s.ex[l] = bex;
-    s_end[l] = bex * WAVE + len;
-    const OffT offp = start - (OffT)bex * WAVE;
+    s_end[l] = bex * BW + lead + len;
+    // B index of the entry's block 0 (even with PER 2, its low bit then
+    // carrying `lead`).
+    const OffT offp = start - (OffT)bex * BW;
     if constexpr (PACK) {
       reinterpret_cast<u64*>(s.off)[l] = ((u64)__float_as_uint(av) << 32) | (u32)offp;
     } else {
@@ -410,23 +465,43 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
         bs = s.off[e];
         if constexpr (VAL) ae = s.av[e];
       }
-      bs += (OffT)b * WAVE;                                    // the block's first B index
-      const int rem = lane < nblk ? s_end[e] - b * WAVE : 0;  // its products (> 64: all)
-      for (int j0 = 0; j0 < nblk; j0 += ROWS) {
+      int first = 0;  // PER 2: the block is its entry's first and leads
+      if constexpr (PER == 2) {
+        first = (int)(bs & 1);
+        bs -= (OffT)first;
+        if (s.ex[e] != b) first = 0;
+      }
+      bs += (OffT)b * BW;                                    // the block's first B index
+      const int rem = lane < nblk ? s_end[e] - b * BW : 0;  // its slots (> BW: all)
+      const int remx = PER == 2 ? 2 * rem + first : rem;
+      for (int j0 = 0; j0 < nblk; j0 += ROWS / PER) {
         i32 c[ROWS];
         float v[ROWS], p[ROWS];
         bool ok[ROWS];
+        if constexpr (PER == 1) {
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-          ok[r] = lane < __builtin_amdgcn_readlane(rem, j0 + r);
-          src.load_at(lane_value(bs, j0 + r), ok[r] ? lane_b : 0, c[r], v[r]);
+          for (int r = 0; r < ROWS; ++r) {
+            ok[r] = lane < __builtin_amdgcn_readlane(rem, j0 + r);
+            src.load_at(lane_value(bs, j0 + r), ok[r] ? lane_b : 0, c[r], v[r]);
+          }
+        } else {
+          // Lane l takes slots 2l and 2l + 1 of the block: entries 2r and
+          // 2r + 1 of the lane's arrays are neighbouring products.
+#pragma unroll
+          for (int r = 0; r < ROWS / 2; ++r) {
+            const int x = __builtin_amdgcn_readlane(remx, j0 + r);
+            ok[2 * r] = 2 * lane < (x >> 1) && (lane != 0 || (x & 1) == 0);
+            ok[2 * r + 1] = 2 * lane + 1 < (x >> 1);
+            src.load2_at(lane_value(bs, j0 + r), 2 * lane < (x >> 1) ? lane_b : 0, c[2 * r],
+                         c[2 * r + 1], v[2 * r], v[2 * r + 1]);
+          }
         }
         // All ROWS gathers leave before any result is used.  Without this
         // fence the scheduler moves the first product's use up between the
         // loads, and the wait for it holds back the other seven.
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) p[r] = VAL ? lane_value(ae, j0 + r) * v[r] : 0.f;
+        for (int r = 0; r < ROWS; ++r) p[r] = VAL ? lane_value(ae, j0 + r / PER) * v[r] : 0.f;
         f(c, p, ok);
       }
     }
@@ -537,6 +612,32 @@ __device__ __forceinline__ void lds_add_f32_rows(float* base, const int (&x)[ROW
       do {
         assumed = cur;
         cur = atomicCAS(&q[x[r]], assumed, __float_as_uint(__uint_as_float(assumed) + p[r]));
+      } while (cur != assumed);
+    }
+  }
+}
+
+// As lds_add_f32_rows, with the words given as byte offsets from base.
+template <int ROWS>
+__device__ __forceinline__ void lds_add_f32_rows_at(unsigned char* base, const u32 (&at)[ROWS],
+                                                    const float (&p)[ROWS], const bool (&ok)[ROWS],
+                                                    const u32 (&old)[ROWS]) {
+  u32 got[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    got[r] = old[r];
+    if (ok[r])
+      got[r] = atomicCAS(reinterpret_cast<u32*>(base + at[r]), old[r],
+                         __float_as_uint(__uint_as_float(old[r]) + p[r]));
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    if (ok[r] && got[r] != old[r]) {
+      u32 cur = got[r], assumed;
+      do {
+        assumed = cur;
+        cur = atomicCAS(reinterpret_cast<u32*>(base + at[r]), assumed,
+                        __float_as_uint(__uint_as_float(assumed) + p[r]));
       } while (cur != assumed);
     }
   }
@@ -1032,7 +1133,9 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
 // row, two resident per CU, one bit per output column over a window of
 // SYM_WIN_TILES tiles (2^19 columns, 64 KB of LDS); wider matrices take
 // several windows, and the slab pointers of B restrict every window to its
-// own products, so each product is gathered once.  Records for every tile of
+// own products, so each product is gathered once.  With PAIRS, bcol is a
+// copy of B's columns with a spare entry past the end (k_copy_cols) and the
+// block walk gathers two neighbouring columns per lane.  Records for every tile of
 // the row its entry count and its count of nonzero bitmap words, packed as
 // entries | words << 16 (the long-row numeric kernels' step plan and output
 // offsets), and, when rowbits is set, the row's bitmap in compressed form:
@@ -1042,7 +1145,7 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
 constexpr int SYM_WIN_TILES = 32;
 constexpr int SYM_WIN = SYM_WIN_TILES * TILE_W;
 
-template <bool BATCH, class OffT, int ROWS>
+template <bool BATCH, class OffT, int ROWS, bool PAIRS>
 __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
     RowSet rs, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const i64* __restrict__ brp,
@@ -1067,7 +1170,8 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
   __shared__ i64 s_scr64[WG / WAVE];
   __shared__ i32 s_scr32[WG / WAVE];
   const ChunkLds<OffT> cl{s_off, nullptr, s_ex, s_mark, s_scr};
-  const ColumnEntries src{bcol};
+  using Src = std::conditional_t<PAIRS, ColumnPairs, ColumnEntries>;
+  const Src src{bcol};
   const int l = threadIdx.x;
   for (int d = l; d < ROWS * WG; d += WG) s_mark[d] = 0;
   __shared__ i32 s_slot;
@@ -1084,6 +1188,46 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
       const int nd = (int)((wcols + 31) / 32);
       for (int d = l; d < (nd + 31) / 32 * GW; d += WG) s_bits[d] = 0;
       __syncthreads();
+      auto set_bits = [&](const auto& c, const auto&, const auto& ok) {
+        if constexpr (PAIRS) {
+          // Entries 2r and 2r+1 of a lane are neighbouring products of one
+          // B row in the block walk (often in one word when the row is
+          // dense): when their words match, one atomic sets both bits.
+#pragma unroll
+          for (int r = 0; r < ROWS; r += 2) {
+            const int x0 = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
+            const int x1 = (int)((i64)c[r + 1] - win);
+            const int w0 = x0 >> 5, w1 = x1 >> 5;
+            u32 b0 = ok[r] ? 1u << (x0 & 31) : 0u;
+            u32 b1 = ok[r + 1] ? 1u << (x1 & 31) : 0u;
+            if (w0 == w1) {
+              b0 |= b1;
+              b1 = 0u;
+            }
+            if (b0) lds_or_u32(&s_bits[w0 + (w0 >> 5)], b0);
+            if (b1) lds_or_u32(&s_bits[w1 + (w1 >> 5)], b1);
+          }
+        } else {
+          // Lanes 2k and 2k+1 hold neighbouring products (consecutive
+          // entries of a B row, often in one word when the row is dense):
+          // when their words match, the even lane sets both bits and the
+          // odd lane none, which halves the same-address atomics a run of
+          // one word costs.  (Merging quads as well, through a second
+          // exchange, measured no faster: the added VALU work cost what the
+          // atomics saved.)
+          const bool odd = threadIdx.x & 1;
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) {
+            const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
+            const int w = x >> 5;
+            u32 bit = ok[r] ? 1u << (x & 31) : 0u;
+            const int wp = __builtin_amdgcn_mov_dpp(w, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+            const u32 bp = (u32)__builtin_amdgcn_mov_dpp((int)bit, 0xB1, 0xf, 0xf, false);
+            if (wp == w) bit = odd ? 0u : bit | bp;
+            if (bit) lds_or_u32(&s_bits[w + (w >> 5)], bit);
+          }
+        }
+      };
       for (i64 e0 = a0; e0 < a1; e0 += WG) {
         const i64 e = e0 + l;
         int len = 0;
@@ -1096,26 +1240,7 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
           len = sk[tw1] - s0;
         }
         const int nvalid = (int)min((i64)WG, a1 - e0);
-        // Lanes 2k and 2k+1 hold neighbouring products (consecutive entries
-        // of a B row, often in one word when the row is dense): when their
-        // words match, the even lane sets both bits and the odd lane none,
-        // which halves the same-address atomics a run of one word costs.
-        // (Merging quads as well, through a second exchange, measured no
-        // faster: the added VALU work cost what the atomics saved.)
-        chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl,
-                                        [&](const auto& c, const auto&, const auto& ok) {
-          const bool odd = threadIdx.x & 1;
-#pragma unroll
-          for (int r = 0; r < ROWS; ++r) {
-            const int x = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
-            const int w = x >> 5;
-            u32 bit = ok[r] ? 1u << (x & 31) : 0u;
-            const int wp = __builtin_amdgcn_mov_dpp(w, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
-            const u32 bp = (u32)__builtin_amdgcn_mov_dpp((int)bit, 0xB1, 0xf, 0xf, false);
-            if (wp == w) bit = odd ? 0u : bit | bp;
-            if (bit) lds_or_u32(&s_bits[w + (w >> 5)], bit);
-          }
-        });
+        chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl, set_bits);
       }
       const int ntw = tw1 - tw0;
       // Thread l owns the window's 32-word group l (words past nd are stale
@@ -1477,9 +1602,12 @@ constexpr int SEG_GPT = TILE_W / 32 / 32;            // 32-word groups per tile
 constexpr int SEG_TILE_GT = SEG_GPT * 8;             // group-table bytes per tile
 // Compressed-step layout of the accumulator region (bytes), for n entries in
 // w nonzero words over 16 groups per tile: accumulators [0, 4n); staged
-// words from OW = 4n rounded up to 8 (u64: entry prefix << 47 | word index
-// << 32 | bits); the group table from OW + 8w (u64: first slot << 32 |
-// nonzero-word mask).  A step's tiles cost 4c + 8w + 128 bytes each.
+// words from OW = 4n rounded up to 8 (u64: word index << 48 | byte offset
+// of the word's first accumulator << 32 | bits); the group table from
+// OW + 8w (u64: byte offset of the group's first staged word << 32 |
+// nonzero-word mask).  Offsets count from the region's start, so a product
+// finds its accumulator with two LDS reads and no subtraction of the
+// step's first column.  A step's tiles cost 4c + 8w + 128 bytes each.
 static_assert(SEG_MAXT * TILE_W / 32 <= (1 << 15) && HSEG_LDS / 4 < (1 << 17),
               "staged word fields");
 
@@ -1552,13 +1680,14 @@ __device__ __forceinline__ SegStep seg_plan(const i32* s_tc, const i32* tc, int 
 // tile sit on 16 consecutive lanes, whose scans of (nonzero words,
 // entries), added to the tile's own prefixes (lane j of every wave computes
 // tile j's from the tile counts), give each group's first slot and entry
-// prefix.  Two barriers; s_r is left dirty.  The copied words and masks are
-// read once: non-temporal loads keep them from evicting B's rows from L2.
+// prefix.  cwd_off is s_cwd's byte offset in the region.  Two barriers; s_r
+// is left dirty.  The copied words and masks are read once: non-temporal
+// loads keep them from evicting B's rows from L2.
 template <int WG>
 __device__ __forceinline__ void stage_packed(const u32* __restrict__ pw,
                                              const u32* __restrict__ mk, const i32* s_tc,
                                              const i32* tc, int nt, int t, int t1, int nw,
-                                             u32* s_r, u64* s_cwd, u64* s_gt) {
+                                             u32* s_r, u64* s_cwd, u64* s_gt, u32 cwd_off) {
   static_assert(SEG_MAXT * SEG_GPT <= 2 * WG, "two groups per thread");
   constexpr int CU = 4;  // copy loads in flight per thread
   const int l = threadIdx.x;
@@ -1587,13 +1716,13 @@ __device__ __forceinline__ void stage_packed(const u32* __restrict__ pw,
     for (int q = 0; q < cnt; ++q) ent += __popc(s_r[slot0 + q]);
     u32 epre = (u32)(__shfl(pc, tj) + seg_scan_incl<SEG_GPT>(ent) - ent);
     if (g < ngroups) {
-      s_gt[g] = ((u64)(u32)slot0 << 32) | mask;
+      s_gt[g] = ((u64)(cwd_off + 8u * (u32)slot0) << 32) | mask;
       u32 mm = mask;
       for (int slot = slot0; mm; ++slot) {
         const int j = __ffs(mm) - 1;
         mm &= mm - 1;
         const u32 wv = s_r[slot];
-        s_cwd[slot] = ((u64)epre << 47) | ((u64)(u32)(g * 32 + j) << 32) | wv;
+        s_cwd[slot] = ((u64)((u32)(g * 32 + j) << 16 | 4u * epre) << 32) | wv;
         epre += __popc(wv);
       }
     }
@@ -1637,8 +1766,8 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
   for (int slot = l; slot < nwords; slot += WG) {
     const u64 e = s_cwd[slot];
     u32 bits = (u32)e;
-    int q = (int)(e >> 47);
-    const int x0 = (int)((e >> 32) & 0x7fff) * 32;
+    int q = (int)((e >> 34) & 0x3fff);
+    const int x0 = (int)(e >> 48) * 32;
     while (bits) {
       const int b0 = __ffs(bits) - 1;
       bits &= bits - 1;
@@ -1782,7 +1911,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         dword = (m >> j) & 1u ? bits[wbase + pre + __popc(m & ((1u << j) - 1u))] : 0u;
       } else {
         stage_packed<WG>(bits + wbase, masks + (i64)t * SEG_GPT, s_tc, tc, nt, t, t1, P.w,
-                         reinterpret_cast<u32*>(s_raw), s_cwd, s_gt);
+                         reinterpret_cast<u32*>(s_raw), s_cwd, s_gt, (u32)((4 * n + 7) & ~7));
         // Accumulators [0, n) start at zero: clear the copied words and the
         // earlier staged data they overlap (ordered before the products by
         // the chunk's scan barriers).
@@ -1806,36 +1935,44 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         for (int r = 0; r < ROWS; ++r) old[r] = __float_as_uint(s_acc[x[r]]);
         lds_add_f32_rows<ROWS>(s_acc, x, p, ok, old);
       };
+      // Byte offset of the step's group table, less that of the step's
+      // first column's group (so that column c's group entry is at
+      // gt_base + 8 * (c >> 10); u32 arithmetic wraps).
+      const u32 gt_base = (u32)((4 * n + 7) & ~7) + 8u * (u32)P.w - 8u * (u32)(cbase >> 10);
       auto add_rank = [&](const auto& c, const auto& p, const auto& ok) {
-        // In halves of H rows (register pressure).
+        // In halves of H rows (register pressure).  The step's tiles start
+        // at a multiple of 1024 columns, so a column's bits below 10 place
+        // it in its group's words and its word without subtracting cbase;
+        // bfe takes its field width from the low 5 bits of the operand.
         constexpr int H = ROWS / 2;
 #pragma unroll
         for (int h0 = 0; h0 < ROWS; h0 += H) {
-          int x[H], rank[H];
+          u32 at[H], old[H];
           u64 gg[H], e[H];
-          u32 old[H];
           float ph[H];
           bool okh[H];
 #pragma unroll
           for (int r = 0; r < H; ++r) {
             okh[r] = ok[h0 + r];
             ph[r] = p[h0 + r];
-            x[r] = (int)(c[h0 + r] - cbase);  // a column of the step for every lane
           }
 #pragma unroll
-          for (int r = 0; r < H; ++r) gg[r] = s_gt[x[r] >> 10];  // first slot << 32 | nonzero-word mask
+          for (int r = 0; r < H; ++r)  // first staged word's offset << 32 | nonzero-word mask
+            gg[r] = *reinterpret_cast<const u64*>(s_raw + (gt_base + 8u * ((u32)c[h0 + r] >> 10)));
 #pragma unroll
           for (int r = 0; r < H; ++r) {
-            const int slot = (int)(gg[r] >> 32) +
-                             __popc((u32)gg[r] & ((1u << ((x[r] >> 5) & 31)) - 1u));
-            e[r] = s_cwd[slot];  // entry prefix << 47 | word index << 32 | bits
+            const u32 below = __builtin_amdgcn_ubfe((u32)gg[r], 0u, (u32)c[h0 + r] >> 5);
+            // word index << 48 | first accumulator's offset << 32 | bits
+            e[r] = *reinterpret_cast<const u64*>(s_raw + ((u32)(gg[r] >> 32) + 8u * __popc(below)));
           }
 #pragma unroll
-          for (int r = 0; r < H; ++r)
-            rank[r] = (int)(e[r] >> 47) + __popc((u32)e[r] & ((1u << (x[r] & 31)) - 1u));
+          for (int r = 0; r < H; ++r) {
+            const u32 below = __builtin_amdgcn_ubfe((u32)e[r], 0u, (u32)c[h0 + r]);
+            at[r] = ((u32)(e[r] >> 32) & 0xffffu) + 4u * __popc(below);
+          }
 #pragma unroll
-          for (int r = 0; r < H; ++r) old[r] = __float_as_uint(s_acc[rank[r]]);
-          lds_add_f32_rows<H>(s_acc, rank, ph, okh, old);
+          for (int r = 0; r < H; ++r) old[r] = *reinterpret_cast<const u32*>(s_raw + at[r]);
+          lds_add_f32_rows_at<H>(s_raw, at, ph, okh, old);
         }
       };
       if (inreg) {
@@ -2202,19 +2339,34 @@ void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const 
 }
 
 // int32 B offsets (and eight products per lane in flight) when B's entries
-// are indexable by int32.
+// are indexable by int32.  colpad: B's columns with a spare entry past the
+// end (k_copy_cols), or nullptr; with it the block walk gathers two columns
+// per lane.
 template <bool BATCH>
 void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr& B,
                        const i64* bbase, const i32* slab, i32 nt, i32* tilecnt, i64* nnz,
-                       u32* rowbits, u32* rowmask, i32* next, hipStream_t s) {
-  if (B.nnz < ((i64)1 << 31))
-    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8>), dim3(grid), dim3(HWG), 0, s, rs, A.rp,
-                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, rowmask,
-                       next);
+                       u32* rowbits, u32* rowmask, i32* next, hipStream_t s,
+                       const i32* colpad = nullptr) {
+  if (colpad && B.nnz < ((i64)1 << 31))
+    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8, true>), dim3(grid), dim3(HWG), 0, s, rs,
+                       A.rp, A.col, B.rp, colpad, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits,
+                       rowmask, next);
+  else if (B.nnz < ((i64)1 << 31))
+    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8, false>), dim3(grid), dim3(HWG), 0, s, rs,
+                       A.rp, A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits,
+                       rowmask, next);
   else
-    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i64, 4>), dim3(grid), dim3(HWG), 0, s, rs, A.rp,
-                       A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits, rowmask,
-                       next);
+    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i64, 4, false>), dim3(grid), dim3(HWG), 0, s, rs,
+                       A.rp, A.col, B.rp, B.col, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits,
+                       rowmask, next);
+}
+
+// B's columns with one spare entry, for the two-column block walk.
+void copy_cols(const Csr& B, DBuf<i32>& out, hipStream_t s) {
+  out.alloc((size_t)B.nnz + 1);
+  const unsigned g = (unsigned)std::min<i64>((B.nnz + 256) / 256, (i64)device().cus * 16);
+  hipLaunchKernelGGL(k_copy_cols, dim3(g), dim3(256), 0, s, B.nnz, B.col, out.p);
+  RT_HIP(hipGetLastError());
 }
 
 HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
@@ -2275,10 +2427,14 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     const unsigned grid = (unsigned)std::min<i64>(heavy.n, (i64)device().cus * 2);
     heavy.next.alloc(1);
     zero_slot(heavy.next.p, s);
+    if (B.nnz < ((i64)1 << 31)) {
+      ProfScope ps("copy_cols", s);
+      copy_cols(B, heavy.colpad, s);
+    }
     ProfScope ps("sym_bitmap", s);
     const RowSet rs{heavy.list.p + heavy.off, heavy.n, nullptr, nullptr, 0};
     launch_sym_bitmap<BATCH>(grid, rs, A, B, bbase, heavy.slab.p, heavy.nt, heavy.tilecnt.p, nnz,
-                             heavy.rowbits.p, heavy.rowmask.p, heavy.next.p, s);
+                             heavy.rowbits.p, heavy.rowmask.p, heavy.next.p, s, heavy.colpad.p);
     RT_HIP(hipGetLastError());
   }
 }
@@ -2691,6 +2847,8 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
   DBuf<i32> longq(row_products_queue(A.nnz));
   DevBins sb(m), nb(m);
   DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
+  DBuf<i32> colpad;  // B's columns for the symbolic long-row kernel's pair walk
+  if (B.nnz < ((i64)1 << 31)) colpad.alloc((size_t)B.nnz + 1);
   const i64 nbs = std::max<i64>(1, (m + SCAN_TILE - 1) / SCAN_TILE);
   DBuf<i64> bsum(nbs);
   C.rows = m;
@@ -2716,8 +2874,11 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
                            s, B.rows, nt, B.rp, B.col, slab.p);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p, s);
+      if (colpad.p)
+        hipLaunchKernelGGL(k_copy_cols, dim3(fill_grid(B.nnz + 1)), dim3(256), 0, s, B.nnz, B.col,
+                           colpad.p);
       launch_sym_bitmap<BATCH>(heavy_grid * 2, heavy, A, B, bbase, slab.p, nt, tilecnt.p, nnz.p,
-                               nullptr, nullptr, next.p, s);
+                               nullptr, nullptr, next.p, s, colpad.p);
       hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
       hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, bsum.p, nbs, C.rp + m);
       if (m > 0)
@@ -2790,6 +2951,8 @@ void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
   DBuf<i32> longq(row_products_queue(A.nnz));
   DevBins lb(m), hb(m), sb(m), nb(m);
   DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
+  DBuf<i32> colpad;  // B's columns for the symbolic long-row kernel's pair walk
+  if (B.nnz < ((i64)1 << 31)) colpad.alloc((size_t)B.nnz + 1);
   DBuf<i64> part(scan_partials(m));
   DBuf<i32> next(3);  // row-slot counters: learned tile, symbolic bitmap, numeric tile
   const i64 long_th = std::min(tu.sym[4], tu.num[4]);
@@ -2847,9 +3010,12 @@ void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
       y.nnz = nnz.p;
       launch_hash_bins_dev<MODE_SYM, BATCH>(y, sb, m, tu.shape, s);
       zero_slot(next.p + 1, s);
+      if (colpad.p)
+        hipLaunchKernelGGL(k_copy_cols, dim3(fill_grid(B.nnz + 1)), dim3(256), 0, s, B.nnz, B.col,
+                           colpad.p);
       launch_sym_bitmap<BATCH>(heavy_grid, RowSet{sb.list.p, 0, sb.cnt.p, sb.off.p, 5}, A, B,
                                bbase, slab.p, nt, tilecnt.p, nnz.p, nullptr, nullptr, next.p + 1,
-                               s);
+                               s, colpad.p);
       exclusive_scan_i64(nnz.p, C.rp, m, s, part.p);
       RT_HIP(hipGetLastError());
     });

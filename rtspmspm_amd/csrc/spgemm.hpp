@@ -79,6 +79,7 @@ struct HeavyRows {
   DBuf<u32> rowmask;
   DBuf<i32> next;     // row-slot counter of the symbolic long-row kernel
   DBuf<i32> slab;     // B.rows x (nt+1) tile boundaries of B's rows
+  DBuf<i32> colpad;   // B's columns with a spare entry past the end
 };
 
 // Pipeline pieces.  `key` arrays select and size rows: rows with key 0 are
