@@ -1204,8 +1204,13 @@ __global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
               b0 |= b1;
               b1 = 0u;
             }
-            if (b0) lds_or_u32(&s_bits[w0 + (w0 >> 5)], b0);
-            if (b1) lds_or_u32(&s_bits[w1 + (w1 >> 5)], b1);
+            // Byte offsets (w + w / 32) * 4: one shift and one add-shift.
+            if (b0)
+              lds_or_u32(reinterpret_cast<u32*>(reinterpret_cast<unsigned char*>(s_bits) +
+                                                (((u32)w0 + ((u32)w0 >> 5)) << 2)), b0);
+            if (b1)
+              lds_or_u32(reinterpret_cast<u32*>(reinterpret_cast<unsigned char*>(s_bits) +
+                                                (((u32)w1 + ((u32)w1 >> 5)) << 2)), b1);
           }
         } else {
           // Lanes 2k and 2k+1 hold neighbouring products (consecutive
