@@ -1129,10 +1129,10 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
   return *s_slot;
 }
 
-// LDS-bitmap symbolic kernel for long rows: one workgroup of HWG threads per
-// row, two resident per CU, one bit per output column over a window of
-// SYM_WIN_TILES tiles (2^19 columns, 64 KB of LDS); wider matrices take
-// several windows, and the slab pointers of B restrict every window to its
+// LDS-bitmap symbolic kernel for long rows: one workgroup of WG threads per
+// row (resident: 1024 threads per CU), one bit per output column over a
+// window of WG/16 tiles (2^19 columns and 64 KB of LDS with 512 threads, half
+// that with 256); wider matrices take several windows, and the slab pointers of B restrict every window to its
 // own products, so each product is gathered once.  With PAIRS, bcol is a
 // copy of B's columns with a spare entry past the end (k_copy_cols) and the
 // block walk gathers two neighbouring columns per lane.  Records for every tile of
@@ -1142,17 +1142,24 @@ __device__ __forceinline__ i64 next_slot(i32* next, i32* s_slot) {
 // rowmask gets per tile 16 masks of nonzero words (one per 32-word group),
 // rowbits the nonzero words themselves, packed in column order.  About a
 // quarter of an R-MAT long row's bitmap words are nonzero.
-constexpr int SYM_WIN_TILES = 32;
-constexpr int SYM_WIN = SYM_WIN_TILES * TILE_W;
+// The pair walk runs in workgroups of SYM_PAIR_WG threads, four resident per
+// CU (16-tile windows of 2^18 columns, 40 KB of LDS each): twice the rows in
+// flight per CU hide more of each window's setup and epilogue latency than
+// the 512-thread workgroups' wider windows save (128 threads, eight per CU:
+// slower again).
+constexpr int SYM_PAIR_WG = 256;
 
-template <bool BATCH, class OffT, int ROWS, bool PAIRS>
-__global__ __launch_bounds__(HWG, 4) void k_sym_bitmap(
+template <bool BATCH, class OffT, int ROWS, bool PAIRS, int WG = HWG>
+__global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
     RowSet rs, const i64* __restrict__ arp,
     const i32* __restrict__ acol, const i64* __restrict__ brp,
     const i32* __restrict__ bcol, const i64* __restrict__ bbase, const i32* __restrict__ slab,
     i64 ncols, i32 nt, i32* __restrict__ tilecnt, i64* __restrict__ out_nnz,
     u32* __restrict__ rowbits, u32* __restrict__ rowmask, i32* __restrict__ next) {
-  constexpr int WG = HWG;
+  // A window of one 32-word group per thread: WT tiles.
+  constexpr int WT = WG / (TILE_W / 32 / 32);
+  constexpr int SYM_WIN_TILES = WT;
+  constexpr int SYM_WIN = WT * TILE_W;
   constexpr int WW = SYM_WIN / 32;  // bitmap words per window
   constexpr int GPT = TILE_W / 32 / 32;  // 32-word groups per tile
   static_assert(WW / 32 == WG, "one 32-word group of the window per thread");
@@ -2353,7 +2360,8 @@ void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr&
                        u32* rowbits, u32* rowmask, i32* next, hipStream_t s,
                        const i32* colpad = nullptr) {
   if (colpad && B.nnz < ((i64)1 << 31))
-    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8, true>), dim3(grid), dim3(HWG), 0, s, rs,
+    hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8, true, SYM_PAIR_WG>), dim3(grid * 2),
+                       dim3(SYM_PAIR_WG), 0, s, rs,
                        A.rp, A.col, B.rp, colpad, bbase, slab, B.cols, nt, tilecnt, nnz, rowbits,
                        rowmask, next);
   else if (B.nnz < ((i64)1 << 31))
