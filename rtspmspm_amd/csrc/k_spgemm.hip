@@ -1169,7 +1169,8 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
   // w sits at w + w / 32, so group g starts on bank g mod 32 and the
   // threads reading their own groups in the epilogue do not collide.
   constexpr int GW = 33;  // stored words per group
-  __shared__ u32 s_bits[WW / 32 * GW];
+  // (rounded up to whole 16-byte words, which the clear writes)
+  __shared__ __align__(16) u32 s_bits[(WW / 32 * GW + 3) & ~3];
   __shared__ OffT s_off[WG];
   __shared__ i32 s_ex[WG + 1];
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
@@ -1193,7 +1194,8 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
       const i64 win = (i64)tw0 * TILE_W;
       const i64 wcols = min((i64)SYM_WIN, ncols - win);
       const int nd = (int)((wcols + 31) / 32);
-      for (int d = l; d < (nd + 31) / 32 * GW; d += WG) s_bits[d] = 0;
+      for (int d = l; d < ((nd + 31) / 32 * GW + 3) / 4; d += WG)
+        reinterpret_cast<uint4*>(s_bits)[d] = make_uint4(0u, 0u, 0u, 0u);
       __syncthreads();
       auto set_bits = [&](const auto& c, const auto&, const auto& ok) {
         if constexpr (PAIRS) {
@@ -1255,6 +1257,7 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
         chunk_products<WG, ROWS, false>(len, start, 0.f, nvalid, src, cl, set_bits);
       }
       const int ntw = tw1 - tw0;
+      i64 wsum = 0;  // the window's entries (rowbits path)
       // Thread l owns the window's 32-word group l (words past nd are stale
       // and count as zero): its nonzero-word mask and entry count.  The 16
       // groups of a tile sit on 16 consecutive lanes.
@@ -1287,8 +1290,14 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
         // writes) and written with coalesced stores.  Both are read once by
         // the numeric pass: non-temporal stores keep them from evicting B's
         // rows from L2.
-        int wtot;
-        const int gpos = block_scan_incl<WG>(gcnt, s_scr32, &wtot) - gcnt;
+        // One scan gives the word positions and (high half) the window's
+        // entry count.
+        u64 both;
+        const int gpos =
+            (int)(u32)block_scan_incl<WG>(((u64)(u32)gent << 32) | (u32)gcnt, s_scr, &both) -
+            gcnt;
+        const int wtot = (int)(u32)both;
+        wsum = (i64)(both >> 32);
         if (tl < ntw)
           __builtin_nontemporal_store(gmask, &rowmask[it * ((i64)nt * GPT) + (i64)tw0 * GPT + l]);
         int k = gpos;
@@ -1304,8 +1313,13 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
           if (q >= 0) __builtin_nontemporal_store(s_bits[q], &dst[q]);
         wcarry += wtot;
       }
-      // (ends with a barrier: s_bits is read before the next window clears it)
-      total += block_reduce_sum<WG>((i64)gent, s_scr64);
+      if (rowbits) {
+        total += wsum;
+        __syncthreads();  // s_bits is read before the next window clears it
+      } else {
+        // (ends with a barrier: s_bits is read before the next window clears it)
+        total += block_reduce_sum<WG>((i64)gent, s_scr64);
+      }
     }
     if (l == 0) out_nnz[row] = total;
     __syncthreads();
