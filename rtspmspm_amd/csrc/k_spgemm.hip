@@ -1182,6 +1182,8 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
   const Src src{bcol};
   const int l = threadIdx.x;
   for (int d = l; d < ROWS * WG; d += WG) s_mark[d] = 0;
+  for (int d = l; d < (int)(sizeof(s_bits) / 16); d += WG)
+    reinterpret_cast<uint4*>(s_bits)[d] = make_uint4(0u, 0u, 0u, 0u);
   __shared__ i32 s_slot;
   for (i64 it = next_slot(next, &s_slot); it < nlist; it = next_slot(next, &s_slot)) {
     const i32 row = list[it];
@@ -1194,9 +1196,8 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
       const i64 win = (i64)tw0 * TILE_W;
       const i64 wcols = min((i64)SYM_WIN, ncols - win);
       const int nd = (int)((wcols + 31) / 32);
-      for (int d = l; d < ((nd + 31) / 32 * GW + 3) / 4; d += WG)
-        reinterpret_cast<uint4*>(s_bits)[d] = make_uint4(0u, 0u, 0u, 0u);
-      __syncthreads();
+      // s_bits is all zero here: cleared at the start and left zero by
+      // every window's epilogue.
       auto set_bits = [&](const auto& c, const auto&, const auto& ok) {
         if constexpr (PAIRS) {
           // Entries 2r and 2r+1 of a lane are neighbouring products of one
@@ -1266,9 +1267,13 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
       int gent = 0;
       {
         const int lim = nd - l * 32;
-        const u32* q = s_bits + l * GW;
+        u32* q = s_bits + l * GW;
 #pragma unroll
         for (int j = 0; j < 32; ++j) gw[j] = q[j];
+        // The group's words go back to zero (the packing below writes only
+        // below wtot, and the stores zero what they take from there).
+#pragma unroll
+        for (int j = 0; j < 32; ++j) q[j] = 0u;
 #pragma unroll
         for (int j = 0; j < 32; ++j) {
           if (j >= lim) gw[j] = 0u;
@@ -1310,7 +1315,10 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
         // not merged in L2).
         const int q0 = l - (int)(reinterpret_cast<uintptr_t>(dst) / 4 & 31);
         for (int q = q0; q < wtot; q += WG)
-          if (q >= 0) __builtin_nontemporal_store(s_bits[q], &dst[q]);
+          if (q >= 0) {
+            __builtin_nontemporal_store(s_bits[q], &dst[q]);
+            s_bits[q] = 0u;
+          }
         wcarry += wtot;
       }
       if (rowbits) {
