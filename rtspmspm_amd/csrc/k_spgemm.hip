@@ -1176,7 +1176,6 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
   __shared__ __align__(16) u16 s_mark[ROWS * WG];
   __shared__ u64 s_scr[WG / WAVE];
   __shared__ i64 s_scr64[WG / WAVE];
-  __shared__ i32 s_scr32[WG / WAVE];
   const ChunkLds<OffT> cl{s_off, nullptr, s_ex, s_mark, s_scr};
   using Src = std::conditional_t<PAIRS, ColumnPairs, ColumnEntries>;
   const Src src{bcol};
