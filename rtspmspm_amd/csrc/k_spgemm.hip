@@ -331,8 +331,8 @@ __device__ __forceinline__ int advance_entry(const i32* __restrict__ ex, int lo,
 // its first product and its first 64-product block, and the chunk is walked
 // in one of two ways, chosen per chunk:
 //
-// * Blocks, when whole 64-product blocks leave at most a quarter of the
-//   lanes idle (long runs: the usual case in long rows).  Each run is cut
+// * Blocks, when whole 64-product blocks leave at most 2/5 of the lanes
+//   idle (long runs: the usual case in long rows).  Each run is cut
 //   into blocks of 64 consecutive products of one entry, and wave w takes
 //   an equal contiguous range of blocks.  For a batch of 64 blocks, lane j
 //   finds the owner of block j by binary search over the entries' first
@@ -424,9 +424,11 @@ __device__ __forceinline__ int chunk_products(int len, OffT start, float av, int
     __syncthreads();  // every wave has read s.scr before the next scan writes it
     return 0;
   }
-  // Blocks when their lanes are at least 3/4 busy (1/2 with pairs, whose
-  // block walk costs about half as many instructions per slot).
-  const bool blocks = PER == 1 ? (i64)tot * 4 >= (i64)tb * (3 * BW) : (i64)tot * 2 >= (i64)tb * BW;
+  // Blocks when their lanes are at least 3/5 busy (1/2 with pairs, whose
+  // block walk costs about half as many instructions per slot): a busy block
+  // slot costs about 20 VALU instructions in the numeric kernels, a product
+  // of the owner-mark walk about 36.
+  const bool blocks = PER == 1 ? (i64)tot * 5 >= (i64)tb * (3 * BW) : (i64)tot * 2 >= (i64)tb * BW;
   if (blocks) {
     const int bex = (int)(u32)inc - nb;
     i32* s_end = reinterpret_cast<i32*>(s.mark);
@@ -1155,7 +1157,7 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
     const i32* __restrict__ acol, const i64* __restrict__ brp,
     const i32* __restrict__ bcol, const i64* __restrict__ bbase, const i32* __restrict__ slab,
     i64 ncols, i32 nt, i32* __restrict__ tilecnt, i64* __restrict__ out_nnz,
-    u32* __restrict__ rowbits, u32* __restrict__ rowmask, i32* __restrict__ next) {
+    u32* __restrict__ rowbits, u64* __restrict__ rowmask, i32* __restrict__ next) {
   // A window of one 32-word group per thread: WT tiles.
   constexpr int WT = WG / (TILE_W / 32 / 32);
   constexpr int SYM_WIN_TILES = WT;
@@ -1163,6 +1165,8 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
   constexpr int WW = SYM_WIN / 32;  // bitmap words per window
   constexpr int GPT = TILE_W / 32 / 32;  // 32-word groups per tile
   static_assert(WW / 32 == WG, "one 32-word group of the window per thread");
+  static_assert((WW & (WW - 1)) == 0, "windows of a power of two words");
+  constexpr u32 WWB = __builtin_ctz(WW);  // bits of a word index in the window
   const i32 nlist = rs.count();
   const i32* __restrict__ list = rs.rows();
   // The window's bitmap, with one spare word after every 32-word group: word
@@ -1204,11 +1208,14 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
           // dense): when their words match, one atomic sets both bits.
 #pragma unroll
           for (int r = 0; r < ROWS; r += 2) {
-            const int x0 = (int)((i64)c[r] - win);  // in [0, wcols) by the slab bounds
-            const int x1 = (int)((i64)c[r + 1] - win);
-            const int w0 = x0 >> 5, w1 = x1 >> 5;
-            u32 b0 = ok[r] ? 1u << (x0 & 31) : 0u;
-            u32 b1 = ok[r + 1] ? 1u << (x1 & 31) : 0u;
+            // Windows start at multiples of SYM_WIN, so a product's word in
+            // the window is a bit field of its column (the slab bounds keep
+            // the columns of busy slots in the window) and its bit is the
+            // column's low 5 bits.
+            const u32 w0 = __builtin_amdgcn_ubfe((u32)c[r], 5u, WWB);
+            const u32 w1 = __builtin_amdgcn_ubfe((u32)c[r + 1], 5u, WWB);
+            u32 b0 = ok[r] ? 1u << ((u32)c[r] & 31u) : 0u;
+            u32 b1 = ok[r + 1] ? 1u << ((u32)c[r + 1] & 31u) : 0u;
             if (w0 == w1) {
               b0 |= b1;
               b1 = 0u;
@@ -1216,10 +1223,10 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
             // Byte offsets (w + w / 32) * 4: one shift and one add-shift.
             if (b0)
               lds_or_u32(reinterpret_cast<u32*>(reinterpret_cast<unsigned char*>(s_bits) +
-                                                (((u32)w0 + ((u32)w0 >> 5)) << 2)), b0);
+                                                ((w0 + (w0 >> 5)) << 2)), b0);
             if (b1)
               lds_or_u32(reinterpret_cast<u32*>(reinterpret_cast<unsigned char*>(s_bits) +
-                                                (((u32)w1 + ((u32)w1 >> 5)) << 2)), b1);
+                                                ((w1 + (w1 >> 5)) << 2)), b1);
           }
         } else {
           // Lanes 2k and 2k+1 hold neighbouring products (consecutive
@@ -1303,7 +1310,8 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
         const int wtot = (int)(u32)both;
         wsum = (i64)(both >> 32);
         if (tl < ntw)
-          __builtin_nontemporal_store(gmask, &rowmask[it * ((i64)nt * GPT) + (i64)tw0 * GPT + l]);
+          __builtin_nontemporal_store((u64)gmask | ((u64)(u32)gent << 32),
+                                      &rowmask[it * ((i64)nt * GPT) + (i64)tw0 * GPT + l]);
         int k = gpos;
 #pragma unroll
         for (int j = 0; j < 32; ++j)
@@ -1706,27 +1714,29 @@ __device__ __forceinline__ SegStep seg_plan(const i32* s_tc, const i32* tc, int 
 
 // Stages compressed step [t, t1) (layout above) from the row's compressed
 // bitmap: pw = the step's nw packed nonzero words, mk = its tiles' group
-// masks (16 per tile).  The words are copied, coalesced, to s_r (the
-// accumulator region's first nw words: nw <= n), with every load of the
-// copy and of the masks issued before the first is used; then thread l
-// takes the window's 32-word group g = l (then l + WG): the 16 groups of a
-// tile sit on 16 consecutive lanes, whose scans of (nonzero words,
-// entries), added to the tile's own prefixes (lane j of every wave computes
-// tile j's from the tile counts), give each group's first slot and entry
-// prefix.  cwd_off is s_cwd's byte offset in the region.  Two barriers; s_r
-// is left dirty.  The copied words and masks are read once: non-temporal
-// loads keep them from evicting B's rows from L2.
+// records (16 per tile: nonzero-word mask | entry count << 32).  The words
+// are copied, coalesced, to s_r (the accumulator region's first nw words:
+// nw <= n), with every load of the copy and of the records issued before
+// the first is used; then thread l takes the window's 32-word group g = l
+// (then l + WG): the 16 groups of a tile sit on 16 consecutive lanes, whose
+// scans of (nonzero words, entries), added to the tile's own prefixes (lane
+// j of every wave computes tile j's from the tile counts), give each group's
+// first slot and entry prefix, and the thread stages its group's words
+// eight per LDS round trip.  cwd_off is s_cwd's byte offset in the region.
+// Two barriers; s_r is left dirty.  The copied words and records are read
+// once: non-temporal loads keep them from evicting B's rows from L2.
 template <int WG>
 __device__ __forceinline__ void stage_packed(const u32* __restrict__ pw,
-                                             const u32* __restrict__ mk, const i32* s_tc,
+                                             const u64* __restrict__ mk, const i32* s_tc,
                                              const i32* tc, int nt, int t, int t1, int nw,
                                              u32* s_r, u64* s_cwd, u64* s_gt, u32 cwd_off) {
   static_assert(SEG_MAXT * SEG_GPT <= 2 * WG, "two groups per thread");
   constexpr int CU = 4;  // copy loads in flight per thread
+  constexpr int WB = 8;  // group words per LDS round trip
   const int l = threadIdx.x;
   const int lane = l & (WAVE - 1);
   const int ngroups = (t1 - t) * SEG_GPT;
-  const u32 m0 = __builtin_nontemporal_load(&mk[min(l, ngroups - 1)]);
+  const u64 m0 = __builtin_nontemporal_load(&mk[min(l, ngroups - 1)]);
   for (int q0 = 0; q0 < nw; q0 += CU * WG) {  // uniform; unconditional (clamped) loads
     u32 r[CU];
 #pragma unroll
@@ -1741,22 +1751,28 @@ __device__ __forceinline__ void stage_packed(const u32* __restrict__ pw,
   __syncthreads();
   for (int g0 = 0; g0 < ngroups; g0 += WG) {  // uniform
     const int g = g0 + l;
-    const u32 mask = g < ngroups ? (g0 ? __builtin_nontemporal_load(&mk[g]) : m0) : 0u;
-    const int cnt = __popc(mask);
+    const u64 rec = g < ngroups ? (g0 ? __builtin_nontemporal_load(&mk[g]) : m0) : 0ull;
+    const u32 mask = (u32)rec;
+    const int cnt = __popc(mask), ent = (int)(rec >> 32);
     const int tj = (g / SEG_GPT) & (WAVE - 1);  // the group's tile in the step (< 64)
     const int slot0 = __shfl(pw0, tj) + seg_scan_incl<SEG_GPT>(cnt) - cnt;
-    int ent = 0;
-    for (int q = 0; q < cnt; ++q) ent += __popc(s_r[slot0 + q]);
     u32 epre = (u32)(__shfl(pc, tj) + seg_scan_incl<SEG_GPT>(ent) - ent);
     if (g < ngroups) {
       s_gt[g] = ((u64)(cwd_off + 8u * (u32)slot0) << 32) | mask;
       u32 mm = mask;
-      for (int slot = slot0; mm; ++slot) {
-        const int j = __ffs(mm) - 1;
-        mm &= mm - 1;
-        const u32 wv = s_r[slot];
-        s_cwd[slot] = ((u64)((u32)(g * 32 + j) << 16 | 4u * epre) << 32) | wv;
-        epre += __popc(wv);
+      for (int q0 = 0; q0 < cnt; q0 += WB) {
+        u32 wv[WB];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) wv[k] = s_r[slot0 + min(q0 + k, cnt - 1)];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) {
+          if (q0 + k < cnt) {
+            const int j = __ffs(mm) - 1;
+            mm &= mm - 1;
+            s_cwd[slot0 + q0 + k] = ((u64)((u32)(g * 32 + j) << 16 | 4u * epre) << 32) | wv[k];
+            epre += __popc(wv[k]);
+          }
+        }
       }
     }
   }
@@ -1836,7 +1852,7 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
 template <bool BATCH, class OffT, int HC, int ROWS>
 __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     RowSet rs, i32 nt, const i32* __restrict__ tilecnt, const u32* __restrict__ rowbits,
-    const u32* __restrict__ rowmask, const i64* __restrict__ key, i64 th,
+    const u64* __restrict__ rowmask, const i64* __restrict__ key, i64 th,
     const i64* __restrict__ arp, const i32* __restrict__ acol, const float* __restrict__ aval,
     const i64* __restrict__ brp, const uint2* __restrict__ bcv, const i64* __restrict__ bbase,
     const i32* __restrict__ slab, const i64* __restrict__ crp, i32* __restrict__ ccol,
@@ -1874,7 +1890,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
     const i64 a0 = arp[row], a1 = arp[row + 1], nA = a1 - a0;
     const i64 bb = BATCH ? bbase[row] : 0;
     const u32* bits = rowbits + h * row_words;                 // packed nonzero words
-    const u32* masks = rowmask + h * ((i64)nt * SEG_GPT);     // 16 group masks per tile
+    const u64* masks = rowmask + h * ((i64)nt * SEG_GPT);     // 16 group records per tile
     i64 wbase = 0;  // packed words of the row's tiles before the current step
     i64 out = crp[row];
     const bool inreg = nA <= (i64)HC * WG;
@@ -1935,7 +1951,7 @@ __global__ __launch_bounds__(HWG, 4) void k_num_heavy_seg(
         // This thread's word of the tile, from the compressed bitmap: the
         // tile's group masks (in s_ex, free between chunks) locate it among
         // the packed words.
-        if (l < SEG_GPT) s_ex[l] = (i32)masks[(i64)t * SEG_GPT + l];
+        if (l < SEG_GPT) s_ex[l] = (i32)(u32)masks[(i64)t * SEG_GPT + l];
         __syncthreads();
         const int g = l >> 5, j = l & 31;
         int pre = 0;
@@ -2378,7 +2394,7 @@ void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const 
 template <bool BATCH>
 void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr& B,
                        const i64* bbase, const i32* slab, i32 nt, i32* tilecnt, i64* nnz,
-                       u32* rowbits, u32* rowmask, i32* next, hipStream_t s,
+                       u32* rowbits, u64* rowmask, i32* next, hipStream_t s,
                        const i32* colpad = nullptr) {
   if (colpad && B.nnz < ((i64)1 << 31))
     hipLaunchKernelGGL((k_sym_bitmap<BATCH, i32, 8, true, SYM_PAIR_WG>), dim3(grid * 2),
@@ -2435,12 +2451,12 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     // when they fit comfortably in free device memory.
     // (Worst case: every word nonzero.)
     const size_t bm_bytes = (size_t)heavy.n * heavy.nt * (TILE_W / 8);
-    const size_t mask_bytes = (size_t)heavy.n * heavy.nt * (TILE_W / 32 / 32) * sizeof(u32);
+    const size_t mask_bytes = (size_t)heavy.n * heavy.nt * (TILE_W / 32 / 32) * sizeof(u64);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     if (bm_bytes + mask_bytes <= free_b / 3 + pool_cached_bytes() / 3) {
       heavy.rowbits.alloc(bm_bytes / sizeof(u32));
-      heavy.rowmask.alloc(mask_bytes / sizeof(u32));
+      heavy.rowmask.alloc(mask_bytes / sizeof(u64));
     }
     // Tile boundaries of every B row (kept for the numeric pass).
     heavy.slab.alloc((size_t)B.rows * (heavy.nt + 1));

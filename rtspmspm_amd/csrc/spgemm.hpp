@@ -72,11 +72,13 @@ struct HeavyRows {
   DBuf<i32> list;  // all binned rows; the heavy ones start at `off`
   i32 off = 0, n = 0, nt = 1;
   DBuf<i32> tilecnt;  // n x nt: entries | nonzero bitmap words << 16 per tile
-  // Each long row's column set (both may be empty): per tile 16 masks of
-  // its nonzero 32-column words (rowmask, n x nt x 16), and those words
-  // packed in column order (rowbits, n x nt x TILE_W/32 words of room).
+  // Each long row's column set (both may be empty): per tile 16 records of
+  // its 32-word groups, the mask of the group's nonzero 32-column words with
+  // the group's entry count in the high half (rowmask, n x nt x 16), and the
+  // nonzero words packed in column order (rowbits, n x nt x TILE_W/32 words
+  // of room).
   DBuf<u32> rowbits;
-  DBuf<u32> rowmask;
+  DBuf<u64> rowmask;
   DBuf<i32> next;     // row-slot counter of the symbolic long-row kernel
   DBuf<i32> slab;     // B.rows x (nt+1) tile boundaries of B's rows
   DBuf<i32> colpad;   // B's columns with a spare entry past the end
