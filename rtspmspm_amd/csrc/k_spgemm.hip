@@ -1812,20 +1812,27 @@ __device__ __forceinline__ void emit_ranked(const u64* s_cwd, int nwords, int n,
   // Column offsets of the staged words' entries, two set bits per round
   // (a wave runs as many rounds as its densest word needs).
   i32* colx = reinterpret_cast<i32*>(acc);
-  for (int slot = l; slot < nwords; slot += WG) {
-    const u64 e = s_cwd[slot];
-    u32 bits = (u32)e;
-    int q = (int)((e >> 34) & 0x3fff);
-    const int x0 = (int)(e >> 48) * 32;
-    while (bits) {
-      const int b0 = __ffs(bits) - 1;
-      bits &= bits - 1;
-      colx[q] = x0 + b0;
-      if (bits) {
-        colx[q + 1] = x0 + (__ffs(bits) - 1);
+  constexpr int UW = 4;  // staged words read per LDS round trip
+  for (int s0 = l; s0 < nwords; s0 += UW * WG) {
+    u64 ew[UW];
+#pragma unroll
+    for (int u = 0; u < UW; ++u) ew[u] = s0 + u * WG < nwords ? s_cwd[s0 + u * WG] : 0ull;
+#pragma unroll
+    for (int u = 0; u < UW; ++u) {
+      const u64 e = ew[u];
+      u32 bits = (u32)e;
+      int q = (int)((e >> 34) & 0x3fff);
+      const int x0 = (int)(e >> 48) * 32;
+      while (bits) {
+        const int b0 = __ffs(bits) - 1;
         bits &= bits - 1;
+        colx[q] = x0 + b0;
+        if (bits) {
+          colx[q + 1] = x0 + (__ffs(bits) - 1);
+          bits &= bits - 1;
+        }
+        q += 2;
       }
-      q += 2;
     }
   }
   __syncthreads();
