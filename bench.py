@@ -69,7 +69,7 @@ def kernel_bytes(name, A, P, cnnz, ncols, TH=TH):
         hi = TH[b] if b < 5 else np.iinfo(np.int64).max
         return (key > lo) & (key <= hi)
 
-    sizes = {"t32": 0, "t128": 1, "t512": 2, "t2048": 3, "t8192": 4}
+    sizes = {"t32": 0, "t128": 1, "t512": 2, "t2048": 3, "t4096": 4, "t8192": 4}
     if name.startswith("num_hash_") or name.startswith("learn_hash_"):
         # learned tables are binned by predicted capacity (~ nnz); nnz bins here
         m = binmask(cnnz, sizes[name.split("_")[-1]])

@@ -117,10 +117,12 @@ std::string tuning_problem(const Tuning& t) {
     if (t.sym[b] < 1 || t.num[b] < 1) return "thresholds must be positive";
     if (b > 0 && (t.sym[b] <= t.sym[b - 1] || t.num[b] <= t.num[b - 1]))
       return "thresholds must increase from bin to bin";
-    if (t.sym[b] > HASH_T[b] / 2 || t.num[b] > HASH_T[b] / 2)
+    if (t.shape[b] < 0 || t.shape[b] > hash_shapes(b))
+      return b == 4 ? "shape of bin 4 must be 0, 1 or 2" : "shape must be 0 or 1";
+    const int slots = hash_slots(b, t.shape[b]);
+    if (t.sym[b] > slots / 2 || t.num[b] > slots / 2)
       return "bin " + std::to_string(b) + " threshold above half its " +
-             std::to_string(HASH_T[b]) + "-slot table";
-    if (t.shape[b] != 0 && t.shape[b] != 1) return "shape must be 0 or 1";
+             std::to_string(slots) + "-slot table";
   }
   if (t.sym[4] > t.num[4]) return "sym[4] must not exceed num[4]";
   if (t.learned_heavy != 0 && t.learned_heavy != 1) return "learned_heavy must be 0 or 1";
@@ -142,7 +144,7 @@ std::string tuning_report(const Tuning& t) {
   std::ostringstream os;
   os << "bins (slots/max nnz/lanes):";
   for (int b = 0; b < 5; ++b)
-    os << ' ' << HASH_T[b] << '/' << t.num[b] << '/' << hash_group(b, t.shape[b]);
+    os << ' ' << hash_slots(b, t.shape[b]) << '/' << t.num[b] << '/' << hash_group(b, t.shape[b]);
   os << "; thresholds sym";
   for (auto v : t.sym) os << ' ' << v;
   os << " num";
@@ -174,6 +176,11 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
   // A later candidate replaces the best when it is faster by more than 1%
   // (noise).
   auto trial = [&](const Tuning& t, const char* what) {
+    if (!tuning_problem(t).empty()) {  // e.g. a 4096 cut with bin 4's half-size table
+      os << (n ? ", " : "") << what << ":skipped";
+      ++n;
+      return;
+    }
     const double ms = eval(t);  // < 0: the candidate's C differed (rejected)
     os << (n ? ", " : "") << what << ":" << (ms < 0 ? std::string("rejected") : std::to_string(ms));
     ++n;
@@ -193,10 +200,14 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
                           {8, 32, 128, 512, 4096},
                           {8, 32, 128, 512, 2048}};
   const char* names[5] = {"default", "cut2048", "cut1024", "load1/4", "load1/4+cut2048"};
+  // The 2048 and 1024 cuts pair with bin 4's 4096-slot table (shape 2:
+  // load factor at most 1/2, as in the other bins) unless the base keeps the
+  // alternate 8192-slot shape; the load-1/4 sets keep 8192 slots.
   auto threshold_sets = [&](const Tuning& base, const char* tag) {
     for (int c = 0; c < 5; ++c) {
       Tuning t = base;
       for (int b = 0; b < 5; ++b) t.sym[b] = t.num[b] = sets[c][b];
+      if (t.shape[4] != 1) t.shape[4] = c == 1 || c == 2 ? 2 : 0;
       const std::string w = std::string(tag) + names[c];
       trial(t, w.c_str());
     }
@@ -205,12 +216,17 @@ Tuning autotune_search(bool learned, const std::function<double(const Tuning&)>&
   if (learned) start.learned_bins = 0;
   threshold_sets(start, "");
   RT_CHECK(best_ms > 0, RT_E_INVALID, "autotune: no candidate produced a valid result");
-  // Workgroup shape, one bin at a time.
+  // Workgroup shape, one bin at a time: every other shape of the bin
+  // (candidates that do not fit the thresholds are skipped).
   for (int b = 0; b < 5; ++b) {
-    Tuning t = best;
-    t.shape[b] ^= 1;
-    const std::string w = "shape" + std::to_string(b) + "=" + std::to_string(t.shape[b]);
-    trial(t, w.c_str());
+    const int cur = best.shape[b];
+    for (int alt = 0; alt <= hash_shapes(b); ++alt) {
+      if (alt == cur) continue;
+      Tuning t = best;
+      t.shape[b] = alt;
+      const std::string w = "shape" + std::to_string(b) + "=" + std::to_string(alt);
+      trial(t, w.c_str());
+    }
   }
   if (learned) {
     // Learned options on top: predictions for the rows of every bin, of

@@ -2364,7 +2364,8 @@ void for_hash_bins(const int shape[5], L&& launch) {
   else launch.template go<64, 512, 256, MODE, BATCH>(2);
   if (shape[3]) launch.template go<256, 2048, 256, MODE, BATCH>(3);
   else launch.template go<128, 2048, 128, MODE, BATCH>(3);
-  if (shape[4]) launch.template go<1024, 8192, 1024, MODE, BATCH>(4);
+  if (shape[4] == 2) launch.template go<256, 4096, 256, MODE, BATCH>(4);
+  else if (shape[4]) launch.template go<1024, 8192, 1024, MODE, BATCH>(4);
   else launch.template go<512, 8192, 512, MODE, BATCH>(4);
 }
 
@@ -2378,18 +2379,20 @@ struct ListedBins {
   hipStream_t s;
   template <int G, int T, int WG, int M, bool BT>
   void go(int b) {
-    launch_hash<G, T, WG, M, BT>(names[b], a, list + off[b], cnt[b], s);
+    launch_hash<G, T, WG, M, BT>(names[T == 4096 ? 5 : b], a, list + off[b], cnt[b], s);
   }
 };
 
 template <int MODE, bool BATCH>
 void launch_hash_bins(const HashArgs& a, const i32* list, const int* cnt, const int* off,
                       const int shape[5], hipStream_t s) {
-  static const char* names[3][5] = {
-      {"sym_hash_t32", "sym_hash_t128", "sym_hash_t512", "sym_hash_t2048", "sym_hash_t8192"},
-      {"num_hash_t32", "num_hash_t128", "num_hash_t512", "num_hash_t2048", "num_hash_t8192"},
+  static const char* names[3][6] = {
+      {"sym_hash_t32", "sym_hash_t128", "sym_hash_t512", "sym_hash_t2048", "sym_hash_t8192",
+       "sym_hash_t4096"},
+      {"num_hash_t32", "num_hash_t128", "num_hash_t512", "num_hash_t2048", "num_hash_t8192",
+       "num_hash_t4096"},
       {"learn_hash_t32", "learn_hash_t128", "learn_hash_t512", "learn_hash_t2048",
-       "learn_hash_t8192"}};
+       "learn_hash_t8192", "learn_hash_t4096"}};
   ListedBins l{a, list, cnt, off, names[MODE], s};
   for_hash_bins<MODE, BATCH>(shape, l);
 }

@@ -45,7 +45,9 @@ struct Tuning {
   // the learned dense-tile kernel (1), or sent to the exact path (0).
   int learned_heavy = 0;
   // Per hash bin, the workgroup shape: 0 the default lanes per row (8, 16,
-  // 64, 128, 512 for bins 0..4), 1 the alternate (16, 32, 32, 256, 1024).
+  // 64, 128, 512 for bins 0..4), 1 the alternate (16, 32, 32, 256, 1024);
+  // bin 4 also has shape 2, 256 lanes per row over a 4096-slot table (for a
+  // long-row cut of at most 2048).
   int shape[5] = {0, 0, 0, 0, 0};
   // Learned method: numeric bins (bit b) whose rows are sized by the
   // predictor; rows predicted into other bins run the exact path.
@@ -54,8 +56,13 @@ struct Tuning {
 // Lanes per row of hash bin b in shape `alt`.
 inline int hash_group(int b, int alt) {
   static const int g[2][5] = {{8, 16, 64, 128, 512}, {16, 32, 32, 256, 1024}};
+  if (b == 4 && alt == 2) return 256;
   return g[alt ? 1 : 0][b];
 }
+// Table slots of hash bin b in shape `alt`.
+inline int hash_slots(int b, int alt) { return b == 4 && alt == 2 ? 4096 : HASH_T[b]; }
+// Largest shape value of hash bin b.
+inline int hash_shapes(int b) { return b == 4 ? 2 : 1; }
 // Empty string when t is usable by the kernels, else what is wrong with it.
 std::string tuning_problem(const Tuning& t);
 

@@ -122,6 +122,29 @@ def test_autotune_records_and_is_used(tmp_path, monkeypatch):
     assert rt.tune_cache_check(str(cache)) == (3, 0)
 
 
+def test_bin4_half_size_table_at_parity(tmp_path, monkeypatch):
+    """Bin 4's 4096-slot table (shape 2, 256 lanes per row, long-row cut
+    2048), forced through a tuning-cache line for both methods: the hash rows
+    of 1024-2048 entries (symbolic and numeric) and the long rows beside
+    them match the oracle."""
+    A = rt.gen_rmat(13, 16, seed=63)
+    ref = oracle.spgemm_f64(A, A)
+    nnz = np.diff(ref.row_ptr)
+    assert ((nnz > 1024) & (nnz <= 2048)).sum() > 1000 and (nnz > 2048).sum() > 1000
+    dA = rt.to_device(A)
+    monkeypatch.setenv("RTSPMSPM_TUNE_CACHE", str(tmp_path / "probe.txt"))
+    key = rt.autotune(dA, dA, mode="plain", reps=1).split()[1]
+    forced = tmp_path / "forced.txt"
+    forced.write_text("".join(
+        f"v2 {key[:-1]}{m} 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 2 31\n"
+        for m in "PL"))
+    assert rt.tune_cache_check(str(forced)) == (2, 0)
+    monkeypatch.setenv("RTSPMSPM_TUNE_CACHE", str(forced))
+    for mode in ("plain", "learned"):
+        C, _ = rt.spmspm(A, A, mode=mode, use_tuned=True)
+        check_parity(C, ref)
+
+
 def test_ae_driver_writes_harness_tables(tmp_path):
     """The experiment driver: per-run latencies, mean/median table, speedup
     table and JSON summary, with the ray ablation breakdown."""

@@ -174,7 +174,10 @@ def test_mtx_ref_header_reproduces_printresult_overwrite(tmp_path):
 
 def test_tune_cache_lines_are_validated(tmp_path):
     good = "v2 r20c20a4b4SP 16 64 256 1024 2048 16 64 256 1024 2048 0 0 1 0 0 1 31"
+    good4 = "v2 r20c20a4b4SL 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 2 31"
     bad = [
+        "v2 k 16 64 256 1024 4096 16 64 256 1024 4096 0 0 0 0 0 2 31",     # 4096 > 4096/2
+        "v2 k 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 2 0 31",     # shape 2 in bin 3
         "r20c20a4b4SP 16 64 256 1024 2048 16 64 256 1024 2048 0",        # old format, no version
         "v1 r20c20a4b4SP 16 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0 31",
         "v2 k 17 64 256 1024 2048 17 64 256 1024 2048 0 0 0 0 0 0 31",     # 17 > 32/2
@@ -190,8 +193,8 @@ def test_tune_cache_lines_are_validated(tmp_path):
         "v2 k sixteen 64 256 1024 2048 16 64 256 1024 2048 0 0 0 0 0 0 31",
     ]
     p = tmp_path / "tune.txt"
-    p.write_text("# comment\n\n" + "\n".join([good] + bad) + "\n")
-    assert rt.tune_cache_check(str(p)) == (1, len(bad))
+    p.write_text("# comment\n\n" + "\n".join([good, good4] + bad) + "\n")
+    assert rt.tune_cache_check(str(p)) == (2, len(bad))
     assert rt.tune_cache_check(str(tmp_path / "absent.txt")) == (0, 0)
 
 
