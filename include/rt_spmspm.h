@@ -216,12 +216,14 @@ int rt_device_info(char* name, int64_t name_len, int64_t* total_mem, int32_t* cu
 
 /* ---- autotuner ------------------------------------------------------------
  * Times a set of candidate row-bin settings (bin thresholds, long-row cut,
- * learned handling of long rows) for `mode` (RT_MODE_PLAIN or
- * RT_MODE_LEARNED) on the given device operands, `reps` timed runs each,
- * and records the fastest for the operands' matrix class in the tuning
- * cache (RTSPMSPM_TUNE_CACHE or ~/.cache/rtspmspm/tune.txt).  Later calls
- * with rt_opts.use_tuned = 1 use it.  desc receives a one-line summary
- * ending in "thresholds sym <5> num <5> learned_heavy <0|1>".
+ * lanes per row of each hash bin and the last bin's table size, learned
+ * handling of rows) for `mode` (RT_MODE_PLAIN or RT_MODE_LEARNED) on the
+ * given device operands, `reps` timed runs each, and records the fastest
+ * for the operands' matrix class in the tuning cache (RTSPMSPM_TUNE_CACHE or
+ * ~/.cache/rtspmspm/tune.txt).  Later calls with rt_opts.use_tuned = 1 use
+ * it.  desc receives a one-line summary: the class, each candidate's time,
+ * the chosen bins as slots/max nnz/lanes, and "thresholds sym <5> num <5>
+ * learned_heavy <0|1> shapes <5> learned_bins <mask>".
  * No counterpart in the reference (its pipeline is fixed at build time).   */
 int rt_autotune(const rt_csr* A, const rt_csr* B, int32_t mode, int32_t reps, char* desc,
                 int64_t desc_len);
