@@ -645,6 +645,86 @@ __device__ __forceinline__ void lds_add_f32_rows_at(unsigned char* base, const u
   }
 }
 
+// Sorts the n packed entries buf[0..n) of a whole-workgroup row (column
+// << 32 | value bits; columns unique, all in [cmn, cmn + span)) by column,
+// in place: least-significant-digit radix passes of 7 bits over
+// column - cmn (three passes for a span of 2^20 columns).  A pass loads
+// entry p = i * WG + t into thread t (i < E = ceil(n / WG) <= EMAX), ranks
+// it among the same-digit entries of its wave from seven ballots, has the
+// digit's first lane record the wave's count for block (i, wave), and one
+// thread per digit turns the counts into exclusive prefixes in (i, wave)
+// order; the digits' totals are scanned into bucket starts, so every entry's
+// new position keeps the order of the pass before (stable), and the entries
+// are written back.  cells holds EMAX x (WG / 64) x 128 u16 counts, start
+// 128 ints.  Four barriers per pass (two in the scan) and none between
+// passes' loads and the previous pass's stores beyond those.  Replaces a
+// bitonic sort of 55-66 barrier-separated stages for rows of 1024-4096
+// entries.
+template <int WG, int EMAX>
+__device__ __forceinline__ void radix_sort_row(u64* buf, int n, i32 cmn, u32 span, u16* cells,
+                                               i32* start, i32* scr) {
+  constexpr int NW = WG / WAVE;
+  constexpr int DB = 7, ND = 1 << DB;
+  static_assert(WG >= ND, "one thread per digit in the scan");
+  const int t = threadIdx.x;
+  const int w = t / WAVE;
+  const int E = (n + WG - 1) / WG;  // uniform, <= EMAX
+  const int bits = n <= 1 || span <= 1u ? 0 : 32 - __clz((int)(span - 1u));
+  u32* cells32 = reinterpret_cast<u32*>(cells);
+  for (int shift = 0; shift < bits; shift += DB) {
+    for (int q = t; q < E * NW * (ND / 2); q += WG) cells32[q] = 0u;
+    u64 x[EMAX];
+    int dr[EMAX];  // digit | rank in the wave << DB
+#pragma unroll
+    for (int i = 0; i < EMAX; ++i) {
+      const int p = i * WG + t;
+      x[i] = i < E && p < n ? buf[p] : ~0ull;
+    }
+    __syncthreads();  // cells zeroed; every entry loaded before any is stored
+#pragma unroll
+    for (int i = 0; i < EMAX; ++i) {
+      dr[i] = 0;
+      if (i < E) {  // uniform
+        const bool v = i * WG + t < n;
+        const int d = v ? (int)((((u32)(x[i] >> 32) - (u32)cmn) >> shift) & (ND - 1)) : 0;
+        u64 peers = __ballot(v);
+#pragma unroll
+        for (int b = 0; b < DB; ++b) {
+          const u64 bb = __ballot(v && ((d >> b) & 1));
+          peers &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        const int r = (int)__builtin_amdgcn_mbcnt_hi((u32)(peers >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((u32)peers, 0u));
+        dr[i] = d | r << DB;
+        if (v && r == 0) cells[(i * NW + w) * ND + d] = (u16)__popcll(peers);
+      }
+    }
+    __syncthreads();
+    int tot = 0;
+    if (t < ND) {
+      for (int c = 0; c < E * NW; ++c) {
+        u16* q = &cells[c * ND + t];
+        const int v = *q;
+        *q = (u16)tot;
+        tot += v;
+      }
+    }
+    int all;
+    const int incl = block_scan_incl<WG>(tot, scr, &all);
+    if (t < ND) start[t] = incl - tot;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < EMAX; ++i) {
+      if (i < E && i * WG + t < n) {
+        const int d = dr[i] & (ND - 1);
+        const int pos = start[d] + cells[(i * NW + w) * ND + d] + (dr[i] >> DB);
+        buf[min(pos, n - 1)] = x[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // LDS-hash row kernel.  A group of G lanes owns one output row and a table of
 // T slots (linear probing, load factor <= 1/2 by binning).  The products of
 // the row are enumerated flat: a chunk of up to G entries A(i,k) is loaded,
@@ -760,11 +840,13 @@ __device__ __forceinline__ void group_minmax(i32& mn, i32& mx, i32* scr) {
 
 template <int G, int T, int WG, int MODE, bool BATCH>
 // A 512-thread workgroup keeps two resident per CU: four waves per SIMD
-// (the second bound), so at most 128 VGPRs.  Groups within a wave holding
+// (the second bound), so at most 128 VGPRs; so do the other whole-workgroup
+// rows (their radix sort would otherwise take the 2048-slot bin past 128
+// and cost it a wave per SIMD).  Groups within a wave holding
 // at most 8 table slots per lane keep seven waves per SIMD (at most 72
 // VGPRs): the learned mode's extra state would otherwise cost it a wave per
 // SIMD against the exact numeric mode (measured 10% on its 512-slot bin).
-__global__ __launch_bounds__(WG, WG >= 512 ? 4 : (G <= WAVE && T / G <= 8 ? 7 : 1))
+__global__ __launch_bounds__(WG, WG >= 512 || G == WG ? 4 : (G <= WAVE && T / G <= 8 ? 7 : 1))
 void k_hash(HashArgs a) {
   static_assert(G <= WAVE || G == WG, "a multi-wave group must be the whole workgroup");
   constexpr bool NUMERIC = MODE != MODE_SYM;
@@ -1081,24 +1163,34 @@ void k_hash(HashArgs a) {
 #pragma unroll
     for (int m = 0; m < SPL; ++m)
       if (ent[m] != ~0ull) buf[pos++] = ent[m];
-    int S = 1;
-    while (S < N) S <<= 1;
-    for (int q = N + l; q < S; q += G) buf[q] = ~0ull;
-    group_sync<G>();
-    // ... bitonic sort ascending (columns are unique, so the order is total) ...
-    for (int k = 2; k <= S; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int p = l; p < (S >> 1); p += G) {
-          const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-          const int ixj = i + j;
-          const u64 x = buf[i], y = buf[ixj];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) {
-            buf[i] = y;
-            buf[ixj] = x;
+    if constexpr (WHOLE) {
+      // ... radix-sorted by column (whole-workgroup rows; the value array is
+      // free for the digit counts) ...
+      static_assert(T / (2 * WG) * (WG / WAVE) * 256 + 512 <= 4 * T, "radix counts fit the values");
+      group_sync<G>();
+      radix_sort_row<WG, T / (2 * WG)>(
+          buf, N, cmn, (u32)(cmx - cmn) + 1u, reinterpret_cast<u16*>(vals),
+          reinterpret_cast<i32*>(reinterpret_cast<unsigned char*>(vals) + 2 * T), s_scr);
+    } else {
+      int S = 1;
+      while (S < N) S <<= 1;
+      for (int q = N + l; q < S; q += G) buf[q] = ~0ull;
+      group_sync<G>();
+      // ... bitonic sort ascending (columns are unique, so the order is total) ...
+      for (int k = 2; k <= S; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int p = l; p < (S >> 1); p += G) {
+            const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+            const int ixj = i + j;
+            const u64 x = buf[i], y = buf[ixj];
+            const bool up = (i & k) == 0;
+            if ((x > y) == up) {
+              buf[i] = y;
+              buf[ixj] = x;
+            }
           }
+          group_sync<G>();
         }
-        group_sync<G>();
       }
     }
     // ... and write the row.
