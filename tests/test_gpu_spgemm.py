@@ -263,6 +263,36 @@ def test_hash_rows_over_narrow_windowed_and_wide_spans(mode):
     assert np.array_equal(Cg.row_ptr, C.row_ptr) and np.array_equal(Cg.col, C.col)
 
 
+def _radix_operands(seed, ncols):
+    """B: 2,000 rows of 32 entries at uniform columns over ncols; A: 600 rows
+    of 10-60 entries into B, so C's rows hold about 320-1,920 entries spread
+    over the whole column range."""
+    rng = np.random.default_rng(seed)
+    nb, per = 2000, 32
+    br = np.repeat(np.arange(nb), per)
+    bc = np.concatenate([rng.choice(ncols, per, replace=False) for _ in range(nb)])
+    B = csr_from_coo(nb, ncols, br, bc, rng.uniform(-1, 1, br.size))
+    lens = rng.integers(10, 61, 600)
+    ar = np.repeat(np.arange(lens.size), lens)
+    ac = np.concatenate([rng.choice(nb, n, replace=False) for n in lens])
+    A = csr_from_coo(lens.size, nb, ar, ac, rng.uniform(-1, 1, ar.size))
+    return A, B
+
+
+@pytest.mark.parametrize("ncols", [1 << 22, (1 << 24) - 3])
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_hash_rows_radix_sorted_over_wide_spans(ncols, mode):
+    """Rows of the 2048- and 8192-slot hash bins whose columns span 2^22-2^24
+    columns, far more than 32 x their tables: they are ordered by the radix
+    sort with four 7-bit digit passes (R-MAT's spans of 2^20 take three)."""
+    A, B = _radix_operands(81, ncols)
+    ref = oracle.spgemm_f64(A, B)
+    nnz = np.diff(ref.row_ptr)
+    assert ((nnz > 256) & (nnz <= 1024)).sum() > 100 and ((nnz > 1024) & (nnz <= 2048)).sum() > 100
+    C, _ = run(A, B, mode=mode)
+    check_parity(C, ref)
+
+
 def test_structure_is_deterministic():
     A = rt.gen_rmat(13, 16, seed=8)
     C1, _ = run(A, A)
