@@ -732,8 +732,9 @@ __device__ __forceinline__ void radix_sort_row(u64* buf, int n, i32 cmn, u32 spa
 // of the chunk, locating its A entry by binary search over the prefix sums.
 // Consecutive lanes therefore read consecutive entries of one B row.
 //   MODE_SYM  : count distinct columns -> a.nnz[row].
-//   MODE_NUM  : accumulate values, compact, bitonic-sort by column and write
-//               the row at a.crp[row] of (a.ccol, a.cval).
+//   MODE_NUM  : accumulate values, order the row by column (a bitmap over
+//               its span, or a sort: radix for whole-workgroup rows, bitonic
+//               within a wave) and write it at a.crp[row] of (a.ccol, a.cval).
 //   MODE_LEARN: as MODE_NUM, but the row's size is a predicted capacity
 //               a.cap[row] (the table has T >= 2*cap slots) and the row is
 //               written at a.crp[row] of a temporary buffer; a row whose
