@@ -811,6 +811,9 @@ struct HashArgs {
   const i32* doff;
   int bin;
   i32* ticket;
+  // B's column count (0: unknown).  Symbolic rows count their columns in a
+  // bitmap instead of a hash table when every column fits the table's bits.
+  i64 bcols;
 };
 constexpr int TICKET_RUN = 32;
 
@@ -890,6 +893,10 @@ void k_hash(HashArgs a) {
 
   const int g = threadIdx.x / G;
   const int l = threadIdx.x & (G - 1);
+  // Symbolic mode over B columns below 32*T: the table's T words are a
+  // bitmap of the row's columns (one LDS OR per product, no probing) and the
+  // count is its population.  Uniform over the launch.
+  const bool bmp = MODE == MODE_SYM && a.bcols > 0 && a.bcols <= (i64)32 * T;
   i32* keys = s_keys + g * T;
   float* vals = NUMERIC ? s_vals + g * T : nullptr;
   const int gb = g * G;
@@ -939,7 +946,7 @@ void k_hash(HashArgs a) {
     if constexpr (MODE == MODE_LEARN) cap = a.cap[row];
   }
   for (int s = l; s < T; s += G) {
-    keys[s] = EMPTY_KEY;
+    keys[s] = bmp ? 0 : EMPTY_KEY;
     if constexpr (NUMERIC) vals[s] = 0.f;
   }
   if constexpr (MODE == MODE_LEARN) {
@@ -952,6 +959,16 @@ void k_hash(HashArgs a) {
     // is read before any is used.
     static_assert(HR == 2 || HR == 4, "named first probes");
     auto ins = [&](const auto& c, const auto& p, const auto& ok) {
+      if constexpr (MODE == MODE_SYM) {
+        if (bmp) {
+#pragma unroll
+          for (int r = 0; r < HR; ++r)
+            if (ok[r])
+              lds_or_u32(reinterpret_cast<u32*>(keys) + (((u32)c[r] >> 5) & (T - 1)),
+                         1u << ((u32)c[r] & 31u));
+          return;
+        }
+      }
       i32* ovf = MODE == MODE_LEARN ? &s_ovf[g] : nullptr;
       const u32 h0 = hash_col<LT>(c[0]), h1 = hash_col<LT>(c[1]);
       if constexpr (HR == 4) {
@@ -1036,6 +1053,16 @@ void k_hash(HashArgs a) {
         cu[u] = bcol[idx[u]];
         if constexpr (NUMERIC) bvu[u] = bval[idx[u]];
       }
+      if constexpr (MODE == MODE_SYM) {
+        if (bmp) {
+#pragma unroll
+          for (int u = 0; u < HU; ++u)
+            if (okp[u])
+              lds_or_u32(reinterpret_cast<u32*>(keys) + (((u32)cu[u] >> 5) & (T - 1)),
+                         1u << ((u32)cu[u] & 31u));
+          continue;
+        }
+      }
 #pragma unroll
       for (int u = 0; u < HU; ++u) {
         if (!okp[u]) continue;
@@ -1049,6 +1076,12 @@ void k_hash(HashArgs a) {
   }
 
   if constexpr (!NUMERIC) {
+    if (bmp) {
+      group_sync<G>();  // every bit is set
+      mycnt = 0;
+#pragma unroll
+      for (int m = 0; m < SPL; ++m) mycnt += __popc((u32)keys[l + m * G]);
+    }
     int tot;
     if constexpr (G <= WAVE) tot = seg_reduce_sum<G>(mycnt);
     else tot = block_reduce_sum<WG>(mycnt, s_scr);
@@ -2532,6 +2565,7 @@ HashArgs make_args(const Csr& A, const Csr& B, const i64* bbase) {
   a.bcol = B.col;
   a.bval = B.val;
   a.bbase = bbase;
+  a.bcols = B.cols;
   return a;
 }
 

@@ -293,6 +293,34 @@ def test_hash_rows_radix_sorted_over_wide_spans(ncols, mode):
     check_parity(C, ref)
 
 
+@pytest.mark.parametrize("ncols", [16384, 65536, 65537, 262144])
+@pytest.mark.parametrize("mode", ["plain", "learned"])
+def test_symbolic_bitmap_rows_at_column_limits(ncols, mode):
+    """Symbolic hash rows count their columns in a bitmap of the table's
+    32 x T bits when every B column fits it (65,536 columns for the 2048-slot
+    bin, 262,144 for the 8192-slot one) and hash them otherwise: B's row 0,
+    gathered by every A row, holds columns 0 and ncols - 1, so every row
+    reaches both ends of the bitmap (or just misses the limit, at 65,537)."""
+    rng = np.random.default_rng(ncols)
+    nb, per = 2000, 32
+    br = np.repeat(np.arange(nb), per)
+    cols = [np.concatenate([[0, ncols - 1], rng.choice(np.arange(1, ncols - 1), per - 2,
+                                                       replace=False)])]
+    cols += [rng.choice(ncols, per, replace=False) for _ in range(nb - 1)]
+    B = csr_from_coo(nb, ncols, br, np.concatenate(cols), rng.uniform(-1, 1, br.size))
+    lens = rng.integers(10, 61, 600)
+    ar = np.repeat(np.arange(lens.size), lens)
+    ac = np.concatenate([np.concatenate([[0], rng.choice(np.arange(1, nb), n - 1, replace=False)])
+                         for n in lens])
+    A = csr_from_coo(lens.size, nb, ar, ac, rng.uniform(-1, 1, ar.size))
+    ref = oracle.spgemm_f64(A, B)
+    nnz = np.diff(ref.row_ptr)
+    assert ((nnz > 256) & (nnz <= 1024)).sum() > 100 and ((nnz > 1024) & (nnz <= 2048)).sum() > 100
+    assert (ref.col[ref.row_ptr[:-1]] == 0).all() and (ref.col[ref.row_ptr[1:] - 1] == ncols - 1).all()
+    C, _ = run(A, B, mode=mode)
+    check_parity(C, ref)
+
+
 def test_structure_is_deterministic():
     A = rt.gen_rmat(13, 16, seed=8)
     C1, _ = run(A, A)
