@@ -897,6 +897,13 @@ void k_hash(HashArgs a) {
   // bitmap of the row's columns (one LDS OR per product, no probing) and the
   // count is its population.  Uniform over the launch.
   const bool bmp = MODE == MODE_SYM && a.bcols > 0 && a.bcols <= (i64)32 * T;
+  // The same condition for exact numeric whole-workgroup rows of up to 2048
+  // slots: the row's columns are ranked through a bitmap and its values
+  // accumulated densely in rank order (two walks over the products, no hash
+  // table).  Larger tables keep the hash: the second path would cost their
+  // kernels register spills.
+  constexpr bool BMN = MODE == MODE_NUM && G == WG && T <= 2048;
+  const bool bmn = BMN && a.bcols > 0 && a.bcols <= (i64)32 * T;
   i32* keys = s_keys + g * T;
   float* vals = NUMERIC ? s_vals + g * T : nullptr;
   const int gb = g * G;
@@ -946,7 +953,7 @@ void k_hash(HashArgs a) {
     if constexpr (MODE == MODE_LEARN) cap = a.cap[row];
   }
   for (int s = l; s < T; s += G) {
-    keys[s] = bmp ? 0 : EMPTY_KEY;
+    keys[s] = bmp || bmn ? 0 : EMPTY_KEY;
     if constexpr (NUMERIC) vals[s] = 0.f;
   }
   if constexpr (MODE == MODE_LEARN) {
@@ -954,6 +961,96 @@ void k_hash(HashArgs a) {
   }
   group_sync<G>();
   int mycnt = 0;
+  if constexpr (BMN) {
+    if (bmn) {
+      // LDS: the key array is the row's column bitmap (later its columns in
+      // rank order); the value array holds each bitmap word's prefix count
+      // (u16, first half) and the row's sums in rank order (second half; a
+      // numeric row has at most T/2 entries).
+      u32* bm = reinterpret_cast<u32*>(keys);
+      u16* wpre = reinterpret_cast<u16*>(vals);
+      float* acc = vals + T / 2;
+      const ChunkLds<i64> cl{s_bs, s_av, s_cex, s_cmark, s_cscr};
+      auto walk = [&](auto val, auto&& f) {
+        constexpr bool V = decltype(val)::value;
+        for (i64 e0 = a0; e0 < a1; e0 += WG) {  // uniform: one row per workgroup
+          const i64 e = e0 + l;
+          int len = 0;
+          i64 bs = 0;
+          float av = 0.f;
+          if (e < a1) {
+            const i64 k = (i64)acol[e] + bb;
+            bs = brp[k];
+            len = (int)(brp[k + 1] - bs);
+            if constexpr (V) av = aval[e];
+          }
+          const int nvalid = (int)min((i64)WG, a1 - e0);
+          if constexpr (V)
+            chunk_products<WG, HR, true>(len, bs, av, nvalid, SplitEntries{bcol, bval}, cl, f);
+          else
+            chunk_products<WG, HR, false>(len, bs, 0.f, nvalid, ColumnEntries{bcol}, cl, f);
+        }
+      };
+      walk(std::false_type{}, [&](const auto& c, const auto&, const auto& ok) {
+#pragma unroll
+        for (int r = 0; r < HR; ++r)
+          if (ok[r]) lds_or_u32(bm + (((u32)c[r] >> 5) & (T - 1)), 1u << ((u32)c[r] & 31u));
+      });
+      __syncthreads();
+      int cnt = 0;
+#pragma unroll
+      for (int j = 0; j < WPL; ++j) cnt += __popc(bm[l * WPL + j]);
+      int N;
+      const int inc = block_scan_incl<WG>(cnt, s_scr, &N);
+      {
+        int run = inc - cnt;
+#pragma unroll
+        for (int j = 0; j < WPL; ++j) {
+          wpre[l * WPL + j] = (u16)run;
+          run += __popc(bm[l * WPL + j]);
+        }
+      }
+      for (int q = l; q < N; q += WG) acc[q] = 0.f;
+      __syncthreads();
+      walk(std::true_type{}, [&](const auto& c, const auto& p, const auto& ok) {
+#pragma unroll
+        for (int r = 0; r < HR; ++r) {
+          if (ok[r]) {
+            const u32 x = (u32)c[r], wi = (x >> 5) & (T - 1);
+            const int rk = wpre[wi] + __popc(bm[wi] & ((1u << (x & 31u)) - 1u));
+            lds_add_f32(&acc[rk], p[r]);
+          }
+        }
+      });
+      __syncthreads();
+      // Columns in rank order: every lane expands its own words.
+      u32 w[WPL];
+#pragma unroll
+      for (int j = 0; j < WPL; ++j) w[j] = bm[l * WPL + j];
+      __syncthreads();  // every word is read before the columns overwrite the bitmap
+      {
+        int run = inc - cnt;
+#pragma unroll
+        for (int j = 0; j < WPL; ++j) {
+          u32 r = w[j];
+          while (r) {
+            const int b = __ffs(r) - 1;
+            r &= r - 1u;
+            bm[run++] = (u32)((l * WPL + j) * 32 + b);
+          }
+        }
+      }
+      __syncthreads();
+      if (active) {
+        const i64 off = a.crp[row];
+        for (int q = l; q < N; q += WG) {
+          a.ccol[off + q] = (i32)bm[q];
+          a.cval[off + q] = acc[q];
+        }
+      }
+      return;
+    }
+  }
   if constexpr (WHOLE) {
     // Batched insert of HR products per lane: the first probe of every row
     // is read before any is used.

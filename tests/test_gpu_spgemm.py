@@ -295,12 +295,14 @@ def test_hash_rows_radix_sorted_over_wide_spans(ncols, mode):
 
 @pytest.mark.parametrize("ncols", [16384, 65536, 65537, 262144])
 @pytest.mark.parametrize("mode", ["plain", "learned"])
-def test_symbolic_bitmap_rows_at_column_limits(ncols, mode):
+def test_bitmap_rows_at_column_limits(ncols, mode):
     """Symbolic hash rows count their columns in a bitmap of the table's
     32 x T bits when every B column fits it (65,536 columns for the 2048-slot
-    bin, 262,144 for the 8192-slot one) and hash them otherwise: B's row 0,
-    gathered by every A row, holds columns 0 and ncols - 1, so every row
-    reaches both ends of the bitmap (or just misses the limit, at 65,537)."""
+    bin, 262,144 for the 8192-slot one), and exact numeric rows of the
+    2048-slot bin rank their columns through such a bitmap and sum in rank
+    order; otherwise both hash.  B's row 0, gathered by every A row, holds
+    columns 0 and ncols - 1, so every row reaches both ends of the bitmap (or
+    just misses the limit, at 65,537)."""
     rng = np.random.default_rng(ncols)
     nb, per = 2000, 32
     br = np.repeat(np.arange(nb), per)
