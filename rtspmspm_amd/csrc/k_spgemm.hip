@@ -275,37 +275,109 @@ __device__ __forceinline__ int bin_of(i64 k, const Thr& th) {
   return 5;
 }
 
-__global__ __launch_bounds__(256) void k_bin_count(i64 n, const i64* __restrict__ x, i64 cap,
-                                                   Thr th, i32* __restrict__ counts) {
-  __shared__ i32 s_cnt[NBIN];
-  if (threadIdx.x < NBIN) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) {
-    const int b = bin_of(min(x[i], cap), th);
-    if (b >= 0) atomicAdd(&s_cnt[b], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < NBIN && s_cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
+// Both binning kernels run a capped grid (bin_grid); workgroup w takes the
+// contiguous run of whole 256-row tiles bin_run(n) * w onwards, so that each
+// bin costs one global atomic per workgroup.  (One workgroup per 256 rows
+// made the bins' counters the bottleneck: 1.08 ms per kernel for 24M rows.)
+__device__ __forceinline__ void bin_run(i64 n, i64& r0, i64& r1) {
+  const i64 per = (((n + gridDim.x - 1) / gridDim.x) + 255) & ~(i64)255;
+  r0 = min(n, (i64)blockIdx.x * per);
+  r1 = min(n, r0 + per);
 }
 
+// This thread's bin counts over the workgroup's run, summed over the
+// workgroup into tot (every thread receives them).
+__device__ __forceinline__ void bin_run_counts(i64 n, const i64* __restrict__ x, i64 cap,
+                                               const Thr& th, i32 (&tot)[NBIN],
+                                               i32 (*s_w)[NBIN]) {
+  i64 r0, r1;
+  bin_run(n, r0, r1);
+  i32 c[NBIN];
+#pragma unroll
+  for (int b = 0; b < NBIN; ++b) c[b] = 0;
+  for (i64 i = r0 + threadIdx.x; i < r1; i += 256) {
+    const int b = bin_of(min(x[i], cap), th);
+#pragma unroll
+    for (int q = 0; q < NBIN; ++q) c[q] += b == q;
+  }
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+  for (int q = 0; q < NBIN; ++q) {
+    i32 v = c[q];
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    if (lane == 0) s_w[w][q] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NBIN; ++q) {
+    tot[q] = 0;
+#pragma unroll
+    for (int v = 0; v < 256 / WAVE; ++v) tot[q] += s_w[v][q];
+  }
+  __syncthreads();  // s_w is free again
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(i64 n, const i64* __restrict__ x, i64 cap,
+                                                   Thr th, i32* __restrict__ counts) {
+  __shared__ i32 s_w[256 / WAVE][NBIN];
+  i32 tot[NBIN];
+  bin_run_counts(n, x, cap, th, tot, s_w);
+  if (threadIdx.x < NBIN) {
+#pragma unroll
+    for (int q = 0; q < NBIN; ++q)
+      if (q == (int)threadIdx.x && tot[q]) atomicAdd(&counts[q], tot[q]);
+  }
+}
+
+// The workgroup reserves its bins' ranges with one atomic each, then writes
+// its rows tile by tile in row order (ranks within a wave from ballots).
 __global__ __launch_bounds__(256) void k_bin_fill(i64 n, const i64* __restrict__ x, i64 cap,
                                                   Thr th, i32* __restrict__ cursor,
                                                   i32* __restrict__ list) {
-  __shared__ i32 s_cnt[NBIN], s_base[NBIN];
-  if (threadIdx.x < NBIN) s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  int b = -1, slot = 0;
-  if (i < n) {
-    b = bin_of(min(x[i], cap), th);
-    if (b >= 0) slot = atomicAdd(&s_cnt[b], 1);
+  __shared__ i32 s_w[256 / WAVE][NBIN];
+  __shared__ i32 s_base[NBIN];
+  i32 tot[NBIN];
+  bin_run_counts(n, x, cap, th, tot, s_w);
+  if (threadIdx.x < NBIN) {
+#pragma unroll
+    for (int q = 0; q < NBIN; ++q)
+      if (q == (int)threadIdx.x) s_base[q] = tot[q] ? atomicAdd(&cursor[q], tot[q]) : 0;
   }
   __syncthreads();
-  if (threadIdx.x < NBIN && s_cnt[threadIdx.x])
-    s_base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], s_cnt[threadIdx.x]);
-  __syncthreads();
-  if (b >= 0) list[s_base[b] + slot] = (i32)i;
+  i64 r0, r1;
+  bin_run(n, r0, r1);
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+  const u64 below = (1ull << lane) - 1ull;
+  for (i64 t0 = r0; t0 < r1; t0 += 256) {  // uniform
+    const i64 i = t0 + threadIdx.x;
+    const int b = i < r1 ? bin_of(min(x[i], cap), th) : -1;
+    int rank = 0;
+#pragma unroll
+    for (int q = 0; q < NBIN; ++q) {
+      const u64 m = __ballot(b == q);
+      if (b == q) rank = (int)__popcll(m & below);
+      if (lane == 0) s_w[w][q] = (i32)__popcll(m);
+    }
+    __syncthreads();
+    if (b >= 0) {
+      int pos = s_base[b] + rank;
+      for (int v = 0; v < w; ++v) pos += s_w[v][b];
+      list[pos] = (i32)i;
+    }
+    __syncthreads();  // every thread has read s_base and s_w
+    if (threadIdx.x < NBIN) {
+      int add = 0;
+#pragma unroll
+      for (int v = 0; v < 256 / WAVE; ++v) add += s_w[v][threadIdx.x];
+      s_base[threadIdx.x] += add;
+    }
+    __syncthreads();
+  }
+}
+
+unsigned bin_grid(i64 n) {
+  return (unsigned)std::max<i64>(1, std::min<i64>((n + 255) / 256, (i64)device().cus * 8));
 }
 
 // ---------------------------------------------------------------------------
@@ -2815,7 +2887,7 @@ void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int c
     counters = own.p;
   }
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, counters, (i64)NBIN, 0);
-  const dim3 grid((unsigned)((n + 255) / 256));
+  const dim3 grid(bin_grid(n));
   if (n > 0) {
     ProfScope ps("bin_count", s);
     hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, counters);
@@ -2986,7 +3058,7 @@ void bin_rows_dev(const i64* x, i64 n, i64 cap, const i64 th_in[5], DevBins& b, 
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.cnt.p, (i64)NBIN, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.ticket.p, (i64)NBIN, 0);
   if (n > 0) {
-    const dim3 grid((unsigned)((n + 255) / 256));
+    const dim3 grid(bin_grid(n));
     hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, b.cnt.p);
     hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, b.cnt.p, b.off.p, b.cursor.p);
     hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, b.cursor.p, b.list.p);
