@@ -147,9 +147,14 @@ struct ColumnPairs {
 };
 
 // B's column indices copied with one spare entry past the end (the
-// ColumnPairs source reads a pair whole).
+// ColumnPairs source reads a pair whole).  With maxp set (captured stages,
+// where the long-row count stays on the device), nothing is copied when no
+// row has more than th products: no row can be a long row.
 __global__ __launch_bounds__(256) void k_copy_cols(i64 n, const i32* __restrict__ col,
-                                                   i32* __restrict__ out) {
+                                                   i32* __restrict__ out,
+                                                   const unsigned long long* __restrict__ maxp,
+                                                   i64 th) {
+  if (maxp && (i64)*maxp <= th) return;
   for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i <= n; i += (i64)gridDim.x * 256)
     out[i] = i < n ? col[i] : 0;
 }
@@ -275,21 +280,26 @@ __device__ __forceinline__ int bin_of(i64 k, const Thr& th) {
   return 5;
 }
 
-// Both binning kernels run a capped grid (bin_grid); workgroup w takes the
-// contiguous run of whole 256-row tiles bin_run(n) * w onwards, so that each
-// bin costs one global atomic per workgroup.  (One workgroup per 256 rows
-// made the bins' counters the bottleneck: 1.08 ms per kernel for 24M rows.)
+// Row binning in three launches with no atomics: k_bin_count writes every
+// workgroup's bin counts over its rows, k_bin_scan (one workgroup) turns
+// them into each workgroup's first slot in every bin (plus the bins' counts
+// and offsets), and k_bin_fill writes each workgroup's rows there, tile by
+// tile in row order, so the lists are deterministic.  Both row kernels run
+// a capped grid (stride_grid); workgroup w takes the contiguous run of whole
+// 256-row tiles from bin_run(n) * w.  (One workgroup per 256 rows with a
+// global atomic per bin and workgroup made the counters the bottleneck:
+// 1.08 ms per kernel for 24M rows; one atomic per bin and workgroup of a
+// capped grid still cost 0.14 ms per kernel, the atomics of all workgroups
+// arriving together on one cache line.)
 __device__ __forceinline__ void bin_run(i64 n, i64& r0, i64& r1) {
   const i64 per = (((n + gridDim.x - 1) / gridDim.x) + 255) & ~(i64)255;
   r0 = min(n, (i64)blockIdx.x * per);
   r1 = min(n, r0 + per);
 }
 
-// This thread's bin counts over the workgroup's run, summed over the
-// workgroup into tot (every thread receives them).
-__device__ __forceinline__ void bin_run_counts(i64 n, const i64* __restrict__ x, i64 cap,
-                                               const Thr& th, i32 (&tot)[NBIN],
-                                               i32 (*s_w)[NBIN]) {
+__global__ __launch_bounds__(256) void k_bin_count(i64 n, const i64* __restrict__ x, i64 cap,
+                                                   Thr th, i32* __restrict__ wgc) {
+  __shared__ i32 s_w[256 / WAVE][NBIN];
   i64 r0, r1;
   bin_run(n, r0, r1);
   i32 c[NBIN];
@@ -309,41 +319,54 @@ __device__ __forceinline__ void bin_run_counts(i64 n, const i64* __restrict__ x,
     if (lane == 0) s_w[w][q] = v;
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NBIN; ++q) {
-    tot[q] = 0;
-#pragma unroll
-    for (int v = 0; v < 256 / WAVE; ++v) tot[q] += s_w[v][q];
-  }
-  __syncthreads();  // s_w is free again
-}
-
-__global__ __launch_bounds__(256) void k_bin_count(i64 n, const i64* __restrict__ x, i64 cap,
-                                                   Thr th, i32* __restrict__ counts) {
-  __shared__ i32 s_w[256 / WAVE][NBIN];
-  i32 tot[NBIN];
-  bin_run_counts(n, x, cap, th, tot, s_w);
   if (threadIdx.x < NBIN) {
+    i32 t = 0;
 #pragma unroll
-    for (int q = 0; q < NBIN; ++q)
-      if (q == (int)threadIdx.x && tot[q]) atomicAdd(&counts[q], tot[q]);
+    for (int v = 0; v < 256 / WAVE; ++v) t += s_w[v][threadIdx.x];
+    wgc[(i64)blockIdx.x * NBIN + threadIdx.x] = t;
   }
 }
 
-// The workgroup reserves its bins' ranges with one atomic each, then writes
-// its rows tile by tile in row order (ranks within a wave from ballots).
+// One workgroup: wgc[w][b] (nwg workgroups' counts) becomes workgroup w's
+// first slot in the whole list for bin b; cnt[b] = bin b's rows, off[b] =
+// its first slot (off[NBIN] = all rows), cursor (if set) = off.
+__global__ __launch_bounds__(256) void k_bin_scan(i32 nwg, i32* __restrict__ wgc,
+                                                  i32* __restrict__ cnt, i32* __restrict__ off,
+                                                  i32* __restrict__ cursor) {
+  __shared__ i32 s_scr[256 / WAVE];
+  const int per = (nwg + 255) / 256;
+  const int w0 = min(nwg, (int)threadIdx.x * per), w1 = min(nwg, w0 + per);
+  i32 run = 0;  // rows of the bins before b
+  for (int b = 0; b < NBIN; ++b) {
+    i32 mine = 0;
+    for (int w = w0; w < w1; ++w) mine += wgc[(i64)w * NBIN + b];
+    i32 tot;
+    const i32 inc = block_scan_incl<256>(mine, s_scr, &tot);
+    i32 at = run + inc - mine;
+    for (int w = w0; w < w1; ++w) {
+      const i32 v = wgc[(i64)w * NBIN + b];
+      wgc[(i64)w * NBIN + b] = at;
+      at += v;
+    }
+    if (threadIdx.x == 0) {
+      cnt[b] = tot;
+      off[b] = run;
+      if (cursor) cursor[b] = run;
+    }
+    run += tot;
+    __syncthreads();  // s_scr is reused by the next bin's scan
+  }
+  if (threadIdx.x == 0) off[NBIN] = run;
+}
+
+// The workgroup's rows, tile by tile in row order, from its first slots in
+// wgc (ranks within a wave from ballots).
 __global__ __launch_bounds__(256) void k_bin_fill(i64 n, const i64* __restrict__ x, i64 cap,
-                                                  Thr th, i32* __restrict__ cursor,
+                                                  Thr th, const i32* __restrict__ wgc,
                                                   i32* __restrict__ list) {
   __shared__ i32 s_w[256 / WAVE][NBIN];
   __shared__ i32 s_base[NBIN];
-  i32 tot[NBIN];
-  bin_run_counts(n, x, cap, th, tot, s_w);
-  if (threadIdx.x < NBIN) {
-#pragma unroll
-    for (int q = 0; q < NBIN; ++q)
-      if (q == (int)threadIdx.x) s_base[q] = tot[q] ? atomicAdd(&cursor[q], tot[q]) : 0;
-  }
+  if (threadIdx.x < NBIN) s_base[threadIdx.x] = wgc[(i64)blockIdx.x * NBIN + threadIdx.x];
   __syncthreads();
   i64 r0, r1;
   bin_run(n, r0, r1);
@@ -376,9 +399,12 @@ __global__ __launch_bounds__(256) void k_bin_fill(i64 n, const i64* __restrict__
   }
 }
 
-unsigned bin_grid(i64 n) {
+// A grid-stride launch over n items: at most 8 workgroups of 256 per CU.
+unsigned stride_grid(i64 n) {
   return (unsigned)std::max<i64>(1, std::min<i64>((n + 255) / 256, (i64)device().cus * 8));
 }
+
+i64 bin_scratch_ints() { return (i64)device().cus * 8 * NBIN + 2 * NBIN + 1; }
 
 // ---------------------------------------------------------------------------
 // Product p of the current chunk -> entry index, from the previous entry
@@ -1637,14 +1663,20 @@ __global__ __launch_bounds__(WG, 4) void k_sym_bitmap(
 }
 
 // Per B row, the offset (relative to the row start) of the first entry in
-// each TILE_W-column tile: slab[k*(nt+1)+t], t = 0..nt.
+// each TILE_W-column tile: slab[k*(nt+1)+t], t = 0..nt.  Grid-stride; with
+// maxp set, nothing is written when no row has more than th products (as
+// k_copy_cols).
 __global__ __launch_bounds__(256) void k_slab_ptrs(i64 brows, i32 nt, const i64* __restrict__ brp,
                                                    const i32* __restrict__ bcol,
-                                                   i32* __restrict__ slab) {
-  const i64 gid = (i64)blockIdx.x * 256 + threadIdx.x;
+                                                   i32* __restrict__ slab,
+                                                   const unsigned long long* __restrict__ maxp,
+                                                   i64 th) {
+  if (maxp && (i64)*maxp <= th) return;
+  const i64 total = brows * (i64)(nt + 1);
+  for (i64 gid = (i64)blockIdx.x * 256 + threadIdx.x; gid < total;
+       gid += (i64)gridDim.x * 256) {
   const i64 k = gid / (nt + 1);
   const int t = (int)(gid - k * (nt + 1));
-  if (k >= brows) return;
   const i64 r0 = brp[k], r1 = brp[k + 1];
   i64 res;
   if (t == 0) {
@@ -1661,6 +1693,7 @@ __global__ __launch_bounds__(256) void k_slab_ptrs(i64 brows, i32 nt, const i64*
     res = lo - r0;
   }
   slab[gid] = (i32)res;
+  }
 }
 
 // Dense LDS column-tile numeric kernel for long rows.  One workgroup per row
@@ -2720,7 +2753,7 @@ void launch_sym_bitmap(unsigned grid, const RowSet& rs, const Csr& A, const Csr&
 void copy_cols(const Csr& B, DBuf<i32>& out, hipStream_t s) {
   out.alloc((size_t)B.nnz + 1);
   const unsigned g = (unsigned)std::min<i64>((B.nnz + 256) / 256, (i64)device().cus * 16);
-  hipLaunchKernelGGL(k_copy_cols, dim3(g), dim3(256), 0, s, B.nnz, B.col, out.p);
+  hipLaunchKernelGGL(k_copy_cols, dim3(g), dim3(256), 0, s, B.nnz, B.col, out.p, nullptr, (i64)0);
   RT_HIP(hipGetLastError());
 }
 
@@ -2769,8 +2802,8 @@ void symbolic_impl(const Csr& A, const Csr& B, const i64* bbase, const i64* key,
     if (B.rows > 0) {
       const i64 nthreads = B.rows * (i64)(heavy.nt + 1);
       ProfScope ps("slab_ptrs", s);
-      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
-                         B.rows, heavy.nt, B.rp, B.col, heavy.slab.p);
+      hipLaunchKernelGGL(k_slab_ptrs, dim3(stride_grid(nthreads)), dim3(256), 0, s, B.rows,
+                         heavy.nt, B.rp, B.col, heavy.slab.p, nullptr, (i64)0);
       RT_HIP(hipGetLastError());
     }
     // Rows that gather the same long B rows run together (their L2 reuse).
@@ -2881,29 +2914,36 @@ void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int c
               int off[7], hipStream_t s, i32* counters) {
   Thr th;
   for (int b = 0; b < 5; ++b) th.t[b] = th_in[b];
+  if (n <= 0) {
+    for (int b = 0; b < NBIN; ++b) cnt[b] = 0;
+    for (int b = 0; b <= NBIN; ++b) off[b] = 0;
+    return;
+  }
   DBuf<i32> own;
   if (!counters) {
-    own.alloc(NBIN);
+    own.alloc((size_t)bin_scratch_ints());
     counters = own.p;
   }
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, counters, (i64)NBIN, 0);
-  const dim3 grid(bin_grid(n));
-  if (n > 0) {
+  // counters: [cnt NBIN | off NBIN + 1 | per-workgroup slots]
+  i32* wgc = counters + 2 * NBIN + 1;
+  const unsigned grid = stride_grid(n);
+  {
     ProfScope ps("bin_count", s);
-    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, counters);
+    hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, n, x, cap, th, wgc);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, (i32)grid, wgc, counters,
+                       counters + NBIN, nullptr);
   }
   RT_HIP(hipGetLastError());
-  RT_HIP(hipMemcpyAsync(cnt, counters, NBIN * sizeof(i32), hipMemcpyDeviceToHost, s));
-  RT_HIP(hipStreamSynchronize(s));
-  off[0] = 0;
-  for (int b = 0; b < NBIN; ++b) off[b + 1] = off[b] + cnt[b];
-  RT_HIP(hipMemcpyAsync(counters, off, NBIN * sizeof(i32), hipMemcpyHostToDevice, s));
-  if (n > 0) {
+  {
     ProfScope ps("bin_fill", s);
-    hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, counters, list);
+    hipLaunchKernelGGL(k_bin_fill, dim3(grid), dim3(256), 0, s, n, x, cap, th, wgc, list);
   }
   RT_HIP(hipGetLastError());
+  int host[2 * NBIN + 1];
+  RT_HIP(hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, s));
   RT_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < NBIN; ++b) cnt[b] = host[b];
+  for (int b = 0; b <= NBIN; ++b) off[b] = host[NBIN + b];
 }
 
 void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipStream_t s,
@@ -2964,8 +3004,8 @@ void learned_heavy_pass(const Csr& A, const Csr& B, const i64* bbase, const i64*
   {
     const i64 nthreads = B.rows * (i64)(nt + 1);
     ProfScope ps("slab_ptrs", s);
-    hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, s,
-                       B.rows, nt, B.rp, B.col, slab.p);
+    hipLaunchKernelGGL(k_slab_ptrs, dim3(stride_grid(nthreads)), dim3(256), 0, s, B.rows, nt,
+                       B.rp, B.col, slab.p, nullptr, (i64)0);
     RT_HIP(hipGetLastError());
   }
   const unsigned grid = (unsigned)std::min<i64>(cnt[5], (i64)device().cus * 2);
@@ -3026,20 +3066,6 @@ void spgemm_plain(const Csr& A, const Csr& B, const Batch& batch, CsrOut& C,
 namespace rt {
 using namespace dev;
 
-// off[b] = exclusive prefix of cnt, cursor[b] = off[b].
-__global__ void k_bin_offsets(const i32* __restrict__ cnt, i32* __restrict__ off,
-                              i32* __restrict__ cursor) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    i32 run = 0;
-    for (int b = 0; b < NBIN; ++b) {
-      off[b] = run;
-      cursor[b] = run;
-      run += cnt[b];
-    }
-    off[NBIN] = run;
-  }
-}
-
 namespace {
 
 unsigned fill_grid(i64 n) {
@@ -3047,23 +3073,25 @@ unsigned fill_grid(i64 n) {
 }
 
 struct DevBins {
-  DBuf<i32> list, cnt, off, cursor, ticket;
+  DBuf<i32> list, cnt, off, cursor, ticket, wgc;
   explicit DevBins(i64 m)
-      : list(std::max<i64>(m, 1)), cnt(NBIN), off(NBIN + 1), cursor(NBIN), ticket(NBIN) {}
+      : list(std::max<i64>(m, 1)), cnt(NBIN), off(NBIN + 1), cursor(NBIN), ticket(NBIN),
+        wgc((size_t)device().cus * 8 * NBIN) {}
 };
 
 void bin_rows_dev(const i64* x, i64 n, i64 cap, const i64 th_in[5], DevBins& b, hipStream_t s) {
   Thr th;
   for (int i = 0; i < 5; ++i) th.t[i] = th_in[i];
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.cnt.p, (i64)NBIN, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, b.ticket.p, (i64)NBIN, 0);
   if (n > 0) {
-    const dim3 grid(bin_grid(n));
-    hipLaunchKernelGGL(k_bin_count, grid, dim3(256), 0, s, n, x, cap, th, b.cnt.p);
-    hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, b.cnt.p, b.off.p, b.cursor.p);
-    hipLaunchKernelGGL(k_bin_fill, grid, dim3(256), 0, s, n, x, cap, th, b.cursor.p, b.list.p);
+    const unsigned grid = stride_grid(n);
+    hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, n, x, cap, th, b.wgc.p);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, (i32)grid, b.wgc.p, b.cnt.p,
+                       b.off.p, b.cursor.p);
+    hipLaunchKernelGGL(k_bin_fill, dim3(grid), dim3(256), 0, s, n, x, cap, th, b.wgc.p, b.list.p);
   } else {
-    hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, b.cnt.p, b.off.p, b.cursor.p);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, s, 0, b.wgc.p, b.cnt.p, b.off.p,
+                       b.cursor.p);
   }
   RT_HIP(hipGetLastError());
 }
@@ -3213,6 +3241,8 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
   HashArgs base = make_args(A, B, bbase);
   const unsigned heavy_grid = (unsigned)device().cus;
   DBuf<i32> next(2);  // row-slot counters of the two long-row kernels
+  // No long row (symbolic or numeric) when no row has more products.
+  const i64 long_th = std::min(tu.sym[4], tu.num[4]);
   double ms_a = 0, ms_b = 0;
   try {
     ms_a = run_captured(s, stage_tag("plain_a", BATCH, m, nt, tu), [&] {
@@ -3225,14 +3255,13 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
       a.nnz = nnz.p;
       launch_hash_bins_dev<MODE_SYM, BATCH>(a, sb, m, tu.shape, s);
       if (B.rows > 0)
-        hipLaunchKernelGGL(k_slab_ptrs,
-                           dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)), dim3(256), 0,
-                           s, B.rows, nt, B.rp, B.col, slab.p);
+        hipLaunchKernelGGL(k_slab_ptrs, dim3(stride_grid(B.rows * (i64)(nt + 1))), dim3(256), 0,
+                           s, B.rows, nt, B.rp, B.col, slab.p, tot.p + 1, long_th);
       const RowSet heavy{sb.list.p, 0, sb.cnt.p, sb.off.p, 5};
       zero_slot(next.p, s);
       if (colpad.p)
         hipLaunchKernelGGL(k_copy_cols, dim3(fill_grid(B.nnz + 1)), dim3(256), 0, s, B.nnz, B.col,
-                           colpad.p);
+                           colpad.p, tot.p + 1, long_th);
       launch_sym_bitmap<BATCH>(heavy_grid * 2, heavy, A, B, bbase, slab.p, nt, tilecnt.p, nnz.p,
                                nullptr, nullptr, next.p, s, colpad.p);
       hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nbs), dim3(1024), 0, s, nnz.p, m, bsum.p);
@@ -3334,8 +3363,8 @@ void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
                               key_heavy.p, key_exact.p, ovf.p, s);
       exclusive_scan_i64(cap.p, tmp_off.p, m, s, part.p);
       if (B.rows > 0)
-        hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)),
-                           dim3(256), 0, s, B.rows, nt, B.rp, B.col, slab.p);
+        hipLaunchKernelGGL(k_slab_ptrs, dim3(stride_grid(B.rows * (i64)(nt + 1))), dim3(256), 0,
+                           s, B.rows, nt, B.rp, B.col, slab.p, tot.p + 1, long_th);
       RT_HIP(hipGetLastError());
     });
     const i64 tmp_total = read_scalar(tmp_off.p + m, s);
@@ -3368,7 +3397,7 @@ void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
       zero_slot(next.p + 1, s);
       if (colpad.p)
         hipLaunchKernelGGL(k_copy_cols, dim3(fill_grid(B.nnz + 1)), dim3(256), 0, s, B.nnz, B.col,
-                           colpad.p);
+                           colpad.p, tot.p + 1, long_th);
       launch_sym_bitmap<BATCH>(heavy_grid, RowSet{sb.list.p, 0, sb.cnt.p, sb.off.p, 5}, A, B,
                                bbase, slab.p, nt, tilecnt.p, nnz.p, nullptr, nullptr, next.p + 1,
                                s, colpad.p);
@@ -3517,7 +3546,7 @@ void spgemm_noalloc(const Csr& A, const Csr& B, i64* crp, i32* ccol, float* cval
   i64* nnz = sc.take<i64>(M);
   i32* slist = sc.take<i32>(M);
   i32* nlist = sc.take<i32>(M);
-  i32* counters = sc.take<i32>(NBIN);
+  i32* counters = sc.take<i32>(bin_scratch_ints());
   i64* partials = sc.take<i64>(scan_partials(m));
   i32* next = sc.take<i32>(2);
   if (!trusted) {
@@ -3542,8 +3571,8 @@ void spgemm_noalloc(const Csr& A, const Csr& B, i64* crp, i32* ccol, float* cval
     slab = sc.take<i32>(std::max<i64>(B.rows, 1) * (nt + 1));
     tilecnt = sc.take<i32>((i64)nlong * nt);
     if (B.rows > 0)
-      hipLaunchKernelGGL(k_slab_ptrs, dim3((unsigned)((B.rows * (i64)(nt + 1) + 255) / 256)),
-                         dim3(256), 0, s, B.rows, nt, B.rp, B.col, slab);
+      hipLaunchKernelGGL(k_slab_ptrs, dim3(stride_grid(B.rows * (i64)(nt + 1))), dim3(256), 0, s,
+                         B.rows, nt, B.rp, B.col, slab, nullptr, (i64)0);
     zero_slot(next, s);
     const RowSet rs{slist + soff[5], nlong, nullptr, nullptr, 0};
     launch_sym_bitmap<false>((unsigned)std::min<i64>(nlong, (i64)device().cus * 2), rs, A, B,

@@ -96,9 +96,11 @@ struct HeavyRows {
 // exact nnz; learned keys are predicted capacities.
 void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipStream_t s,
                   RunStats* st);
-// counters: NBIN device ints of scratch (nullptr: taken from the pool).
+// counters: bin_scratch_ints() device ints of scratch (nullptr: taken from
+// the pool).
 void bin_rows(const i64* x, i64 n, i64 cap, const i64 th[5], i32* list, int cnt[6], int off[7],
               hipStream_t s, i32* counters = nullptr);
+i64 bin_scratch_ints();
 // Reorders the long-row list `rows` (n entries) so that rows gathering the
 // same long B rows run together (k_order.hip).
 void order_long_rows(const Csr& A, const Csr& B, i32* rows, i32 n, hipStream_t s);
