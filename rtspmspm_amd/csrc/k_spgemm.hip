@@ -204,10 +204,42 @@ __global__ __launch_bounds__(256) void k_row_products(
       tmax = max(tmax, sum);
     }
   }
+  // The workgroup's (sum, max) go to its own pair of partials, reduced by
+  // k_row_totals: thousands of workgroups' atomics on one cache line cost
+  // more than the analysis of a small product.
+  __shared__ i64 s_mx[256 / WAVE];
   const i64 mx = seg_reduce_max<WAVE>(tmax);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_mx[threadIdx.x / WAVE] = mx;
   const i64 ws = block_reduce_sum<256>(tsum, s_scr);
-  if ((threadIdx.x & (WAVE - 1)) == 0 && mx > 0) atomicMax(&totals[1], (unsigned long long)mx);
-  if (threadIdx.x == 0 && ws > 0) atomicAdd(&totals[0], (unsigned long long)ws);
+  if (threadIdx.x == 0) {
+    i64 m = 0;
+#pragma unroll
+    for (int w = 0; w < 256 / WAVE; ++w) m = max(m, s_mx[w]);
+    totals[2 + 2 * blockIdx.x] = (unsigned long long)ws;
+    totals[3 + 2 * blockIdx.x] = (unsigned long long)m;
+  }
+}
+
+// totals[0] += the n workgroups' sums, totals[1] = max(totals[1], their
+// maxima), from the partials at totals[2 ..].  One workgroup.
+__global__ __launch_bounds__(256) void k_row_totals(i32 n, unsigned long long* __restrict__ totals) {
+  __shared__ i64 s_scr[256 / WAVE];
+  __shared__ i64 s_mx[256 / WAVE];
+  i64 sum = 0, mx = 0;
+  for (i32 i = threadIdx.x; i < n; i += 256) {
+    sum += (i64)totals[2 + 2 * i];
+    mx = max(mx, (i64)totals[3 + 2 * i]);
+  }
+  mx = seg_reduce_max<WAVE>(mx);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_mx[threadIdx.x / WAVE] = mx;
+  sum = block_reduce_sum<256>(sum, s_scr);
+  if (threadIdx.x == 0) {
+    i64 m = 0;
+#pragma unroll
+    for (int w = 0; w < 256 / WAVE; ++w) m = max(m, s_mx[w]);
+    if (sum > 0) atomicAdd(&totals[0], (unsigned long long)sum);
+    if (m > 0) atomicMax(&totals[1], (unsigned long long)m);
+  }
 }
 
 // The rows of more than RP_LONG A entries (longq[0] of them, listed from
@@ -255,17 +287,24 @@ inline unsigned row_products_grid(i64 nrows) {
 // Entries of the long-row queue of the row analysis (count + rows).
 inline size_t row_products_queue(i64 annz) { return (size_t)(annz / RP_LONG + 2); }
 
-// Row analysis: prod[i] = P_i; totals (zeroed by the caller) += (sum, max).
-// longq holds row_products_queue(A.nnz) entries.  Kernels only (graph
-// capture records them).
+// Entries of the row analysis' totals: (sum, max) and a pair of partials per
+// workgroup of k_row_products.
+inline size_t row_totals_len() { return 2 + 2 * (size_t)device().cus * 16; }
+
+// Row analysis: prod[i] = P_i; totals (row_totals_len() entries, the first
+// two zeroed by the caller) += (sum, max).  longq holds
+// row_products_queue(A.nnz) entries.  Kernels only (graph capture records
+// them).
 template <bool BATCH>
 void launch_row_products(i64 m, const i64* arp, const i32* acol, const i64* brp,
                          const i64* bbase, i64* prod, unsigned long long* totals, i32* longq,
                          hipStream_t s) {
   if (m <= 0) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, longq, (i64)1, 0);
-  hipLaunchKernelGGL(k_row_products<BATCH>, dim3(row_products_grid(m)), dim3(256), 0, s, m, arp,
-                     acol, brp, bbase, prod, totals, longq);
+  const unsigned grid = row_products_grid(m);
+  hipLaunchKernelGGL(k_row_products<BATCH>, dim3(grid), dim3(256), 0, s, m, arp, acol, brp, bbase,
+                     prod, totals, longq);
+  hipLaunchKernelGGL(k_row_totals, dim3(1), dim3(256), 0, s, (i32)grid, totals);
   hipLaunchKernelGGL(k_row_products_long<BATCH>, dim3((unsigned)device().cus * 2), dim3(256), 0,
                      s, longq, arp, acol, brp, bbase, prod, totals);
 }
@@ -2949,7 +2988,7 @@ void bin_rows(const i64* x, i64 n, i64 cap, const i64 th_in[5], i32* list, int c
 void row_products(const Csr& A, const Csr& B, const i64* bbase, i64* prod, hipStream_t s,
                   RunStats* st) {
   const i64 m = A.rows;
-  DBuf<unsigned long long> tot(2);
+  DBuf<unsigned long long> tot(row_totals_len());
   RT_HIP(hipMemsetAsync(tot.p, 0, 2 * sizeof(unsigned long long), s));
   if (m > 0) {
     ProfScope ps("row_products", s);
@@ -3227,7 +3266,7 @@ void plain_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C, c
   RT_CHECK((double)M * nt * 4.0 < 8.0 * (1ull << 30), RT_E_LIMIT,
            "graph mode: tile-count scratch too large; use the direct launch path");
   DBuf<i64> prod(M), nnz(M);
-  DBuf<unsigned long long> tot(2);
+  DBuf<unsigned long long> tot(row_totals_len());
   DBuf<i32> longq(row_products_queue(A.nnz));
   DevBins sb(m), nb(m);
   DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
@@ -3332,7 +3371,7 @@ void learned_graph_impl(const Csr& A, const Csr& B, const i64* bbase, CsrOut& C,
       tmp_off(m + 1);
   DBuf<i32> ovf(M);
   DBuf<float> feat(M * RT_NFEAT), pred(M);
-  DBuf<unsigned long long> tot(2), counts(2);
+  DBuf<unsigned long long> tot(row_totals_len()), counts(2);
   DBuf<i32> longq(row_products_queue(A.nnz));
   DevBins lb(m), hb(m), sb(m), nb(m);
   DBuf<i32> tilecnt((size_t)M * nt), slab((size_t)std::max<i64>(B.rows, 1) * (nt + 1));
@@ -3540,7 +3579,7 @@ void spgemm_noalloc(const Csr& A, const Csr& B, i64* crp, i32* ccol, float* cval
   // Everything but the long rows' pieces is taken (and its size checked)
   // before the first launch.
   int* flag = sc.take<int>(1);
-  unsigned long long* tot = sc.take<unsigned long long>(2);
+  unsigned long long* tot = sc.take<unsigned long long>((i64)row_totals_len());
   i32* longq = sc.take<i32>((i64)row_products_queue(A.nnz));
   i64* prod = sc.take<i64>(M);
   i64* nnz = sc.take<i64>(M);
