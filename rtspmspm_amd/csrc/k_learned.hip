@@ -306,12 +306,14 @@ __global__ __launch_bounds__(256) void k_merge_overflow(i64 m, const i64* __rest
                                                         i64* __restrict__ key_exact,
                                                         unsigned long long* __restrict__ counts) {
   __shared__ i64 s_scr[4];
-  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
   i64 pr = 0, ov = 0;
-  if (i < m) {
-    pr = cap[i] > 0;
-    ov = ovf[i] != 0;
-    if (ov) key_exact[i] = prod[i];
+  // Grid-stride over a capped grid: each workgroup's two counts cost one
+  // atomic each on a shared cache line.
+  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < m; i += (i64)gridDim.x * 256) {
+    const bool o = ovf[i] != 0;
+    pr += cap[i] > 0;
+    ov += o;
+    if (o) key_exact[i] = prod[i];
   }
   pr = block_reduce_sum<256>(pr, s_scr);
   ov = block_reduce_sum<256>(ov, s_scr);
@@ -594,8 +596,10 @@ void learned_capacity_launch(i64 m, const i64* prod, i64 ncols, const float* pre
 void learned_merge_launch(i64 m, const i64* prod, const i64* cap, const i32* ovf, i64* key_exact,
                           unsigned long long* counts, hipStream_t s) {
   if (m <= 0) return;
-  hipLaunchKernelGGL(k_merge_overflow, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, prod,
-                     cap, ovf, key_exact, counts);
+  const unsigned grid =
+      (unsigned)std::max<i64>(1, std::min<i64>((m + 255) / 256, (i64)device().cus * 8));
+  hipLaunchKernelGGL(k_merge_overflow, dim3(grid), dim3(256), 0, s, m, prod, cap, ovf, key_exact,
+                     counts);
   RT_HIP(hipGetLastError());
 }
 
